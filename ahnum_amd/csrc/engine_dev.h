@@ -1,0 +1,57 @@
+// Structures shared by the gfx950 kernels and the host code that launches them.
+#ifndef KITE_ENGINE_DEV_H
+#define KITE_ENGINE_DEV_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kite {
+
+constexpr int kNT = 256;            // threads per work-group
+constexpr int kMaxStages = 16;
+constexpr uint32_t kMaxC = 2048;    // row length (row pass keeps 2 rows x 2 buffers in LDS)
+constexpr uint32_t kMaxR = 1024;    // column length
+constexpr uint32_t kMaxColLds = 96 * 1024;
+constexpr int kMaxLds = 160 * 1024; // gfx950 LDS per work-group
+constexpr int kMaxTop = 64;         // top-region digits handled by the device fold
+
+enum : int { kModeSquare = 0, kModeMul = 1, kModeDigits = 2, kModeRowFft = 3 };
+
+struct Plan {  // radix sequence of one FFT length
+    int n, nst;
+    int rad[kMaxStages];
+};
+
+struct ShapeDev {  // one (R, C, G) shape: H = R*C complex, NG = C/G column groups
+    int R, C, G, NG;
+    int sshift;            // two-level w_H table split: t = hi*2^sshift + lo
+    Plan pr, pc;           // column (length R) and row (length C) plans
+    const double2 *twR;    // w_R^t, t < R
+    const double2 *twC;    // w_C^t, t < C
+    const double2 *tlo;    // w_H^t, t < 2^sshift
+    const double2 *thi;    // w_H^(t 2^sshift)
+};
+
+struct SlotDev {  // one number being operated on (one test in a batch)
+    const ShapeDev *shp;
+    double2 *x;                 // the residue, H complex (after forward column pass)
+    const double2 *y_src;       // multiplication operand (stored form), for kModeRowFft
+    double2 *yfreq;             // operand row spectra, H complex
+    int32_t *dig;               // L digits, blocked [g][r][2G]
+    int64_t *cout;              // carry out of each (g, r) chunk; the last holds the top fold
+    const uint8_t *dcode;       // per element [g][r][l]: ceil and big-digit bits of both digits
+    const double *ta;           // [R] 2^-fa(r), [R] 2^+fa(r)
+    const double *tb;           // [2C] 2^-fb(b), [2C] 2^+fb(b)
+    unsigned long long *maxerr; // max round-off (bits of a non-negative double)
+    double kscale;              // k / (4H)
+    double rscale;              // 1 / R
+    uint64_t k;
+    int b0;                     // digits have b0 or b0+1 bits
+    int top_i0, top_n, top_m, top_shift;  // fold: first top digit within the last chunk, count,
+                                          // m = n - s_t, s_L - s_t
+    uint8_t top_bits[kMaxTop];
+};
+
+}  // namespace kite
+
+#endif

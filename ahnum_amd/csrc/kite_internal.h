@@ -1,0 +1,85 @@
+// Internal definitions of the MI355X kite library (host side).
+//
+// Representation (the k-scaled IBDWT of the reference, src/internal.h:3-17): for N = k*2^n - 1
+// and L real words, T' = n + floor(log2 k) + lam/2^64 (lam = frac(log2 k) to 64 bits),
+// s_j = ceil(j*T'/L), digit j has s_{j+1} - s_j bits, weight w_j = 2^(s_j - j*T'/L) in [1, 2).
+// The stored value is V = x/k (mod N) as sum d_j 2^{s_j}; squaring the weighted convolution
+// and scaling by k/w_j maps V to k*V^2, which is V(x^2). The part at or above 2^n (top digits
+// t..L-1 plus the final carry) is folded with k*2^n == 1 (mod N).
+//
+// Numbers live in HBM as H = L/2 complex doubles in the "after forward column pass" state of
+// the four-step transform (the engine in engine.hip owns that layout).
+#ifndef KITE_INTERNAL_H
+#define KITE_INTERNAL_H
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "kite.h"
+
+namespace kite {
+struct Engine;  // device state of one context (engine.hip)
+}
+
+struct kite_num {
+    void *dev;  // H complex doubles in device memory
+};
+
+struct kite_ctx {
+    uint64_t k = 0, n = 0;
+    int32_t c = -1;
+    uint32_t L = 0;
+    uint32_t plan_order = 0;   // shape variant (0 = tuned/default)
+    std::vector<uint8_t> bits; // bits[j] = s_{j+1} - s_j
+    uint64_t lam = 0;          // frac(log2 k) * 2^64
+    uint64_t sL = 0;           // s_L
+    uint32_t top_t = 0;        // first digit with s_{t+1} > n (L if none)
+    uint64_t top_st = 0;       // s_t
+    uint32_t top_m = 0;        // n - s_t
+    uint32_t b0 = 0;           // floor(T'/L): every digit has b0 or b0+1 bits
+    size_t nlimbs = 0;
+    std::vector<uint64_t> Nl;  // N as limbs
+    uint32_t err_every = 1;
+    kite::Engine *eng = nullptr;
+    std::vector<int64_t> tmpdig;
+};
+
+namespace kite {
+
+// ---- layout (layout.cpp) --------------------------------------------------------------------
+uint64_t frac_log2_64(uint64_t k);
+int build_layout(kite_ctx *ctx);
+// Exact fraction numerator of j*T'/L: (j * T'fix) mod (L*2^64), as a 128-bit value.
+unsigned __int128 phase_num(const kite_ctx *ctx, uint64_t j);
+unsigned __int128 tfix(const kite_ctx *ctx);  // T' * 2^64
+// 2^(-num/(L*2^64)) rounded to double, num < L*2^64.
+double exp2_neg_frac(const kite_ctx *ctx, unsigned __int128 num);
+
+// ---- digits and limbs (layout.cpp) -----------------------------------------------------------
+void digits_to_limbs(kite_ctx *ctx, const int64_t *dig, uint64_t *limbs);
+void limbs_to_digits(kite_ctx *ctx, const uint64_t *limbs, int64_t *dig);
+
+// ---- FFT length rules (layout.cpp) -----------------------------------------------------------
+bool fft_len_ok(uint32_t L);
+
+// ---- engine (engine.hip) ---------------------------------------------------------------------
+int engine_init(kite_ctx *ctx);
+void engine_free(kite_ctx *ctx);
+bool engine_shape_ok(uint32_t L, uint32_t top_digits);
+void *engine_num_alloc(kite_ctx *ctx);
+void engine_num_free(kite_ctx *ctx, void *p);
+void engine_copy(kite_ctx *ctx, void *dst, const void *src);
+void engine_from_digits(kite_ctx *ctx, void *x, const int64_t *dig);
+void engine_to_digits(kite_ctx *ctx, const void *x, int64_t *dig);
+void engine_square(kite_ctx *ctx, void *x, uint64_t count);
+void engine_mul(kite_ctx *ctx, void *x, const void *y);
+void engine_addsub(kite_ctx *ctx, void *x, const void *y, int sign);
+double engine_maxerr(kite_ctx *ctx);
+void engine_reset_maxerr(kite_ctx *ctx);
+void engine_sync(kite_ctx *ctx);
+const char *engine_last_error();
+
+}  // namespace kite
+
+#endif

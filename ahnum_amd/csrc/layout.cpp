@@ -1,0 +1,465 @@
+// Host half of the engine: digit layout and weights, FFT-length choice, balanced carry with the
+// top-region fold, and conversion between digits and limbs.
+//
+// Follows the reference's definitions (src/ctx.c:17-559); the arithmetic is exact (128-bit
+// integers), so the digit boundaries s_j and the weight exponents are bit-identical to the
+// reference's for the same (k, n, L).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "kite_internal.h"
+
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+namespace kite {
+
+// frac(log2 k) as a 64-bit binary fraction: square a 62-fractional-bit mantissa 64 times, each
+// square yielding one bit (src/ctx.c:150-167).
+uint64_t frac_log2_64(uint64_t k) {
+    int e = 63 - __builtin_clzll(k);
+    u128 y = (u128)k << (62 - e);
+    uint64_t f = 0;
+    for (int i = 0; i < 64; i++) {
+        y = (y * y) >> 62;
+        uint64_t top = (uint64_t)(y >> 63);
+        f = (f << 1) | top;
+        y >>= top;
+    }
+    return f;
+}
+
+static inline uint64_t floor_log2(uint64_t k) { return (uint64_t)(63 - __builtin_clzll(k)); }
+
+u128 tfix(const kite_ctx *ctx) { return ((u128)(ctx->n + floor_log2(ctx->k)) << 64) + ctx->lam; }
+
+u128 phase_num(const kite_ctx *ctx, uint64_t j) {
+    // j*T'fix < 2^28 * 2^93: fits 128 bits for every supported (n, L)
+    u128 D = (u128)ctx->L << 64;
+    return ((u128)j * tfix(ctx)) % D;
+}
+
+double exp2_neg_frac(const kite_ctx *ctx, u128 num) {
+    long double f = (long double)num / ((long double)ctx->L * 18446744073709551616.0L);
+    return (double)exp2l(-f);
+}
+
+int build_layout(kite_ctx *ctx) {  // src/ctx.c:198-221
+    const uint32_t L = ctx->L;
+    ctx->lam = frac_log2_64(ctx->k);
+    ctx->bits.assign(L, 0);
+    const u128 T = tfix(ctx), D = (u128)L << 64;
+    ctx->b0 = (uint32_t)(T / D);
+    uint64_t s = 0;
+    ctx->top_t = L;
+    for (uint32_t j = 0; j < L; j++) {
+        uint64_t s1 = (uint64_t)(((u128)(j + 1) * T + D - 1) / D);  // ceil((j+1) T'/L)
+        if (s1 < s || s1 - s > 40) return -1;
+        ctx->bits[j] = (uint8_t)(s1 - s);
+        if (s1 > ctx->n && ctx->top_t == L) {
+            ctx->top_t = j;
+            ctx->top_st = s;
+        }
+        s = s1;
+    }
+    ctx->sL = s;
+    if (ctx->top_t == L) ctx->top_st = s;
+    ctx->top_m = (uint32_t)(ctx->n - ctx->top_st);
+    if (ctx->sL - ctx->top_st > 100 || ctx->top_t == 0) return -1;
+    return 0;
+}
+
+bool fft_len_ok(uint32_t L) {  // src/ctx.c:17-24
+    if (L < 16 || (L & 1)) return false;
+    for (uint32_t p : {2u, 3u, 5u, 7u})
+        while (L % p == 0) L /= p;
+    return L == 1;
+}
+
+// ---- balanced carry and the top fold (src/ctx.c:345-407) --------------------------------------
+
+static inline int64_t split_bal(int64_t v, unsigned b, int64_t *d) {
+    int64_t q = (v + ((int64_t)1 << (b - 1))) >> b;
+    *d = v - (q << b);
+    return q;
+}
+
+static int64_t fold_top(const kite_ctx *ctx, int64_t *dig, int64_t carry) {
+    const uint32_t L = ctx->L, t = ctx->top_t;
+    i128 H = (i128)carry << (ctx->sL - ctx->top_st);
+    unsigned pos = 0;
+    for (uint32_t j = t; j < L; j++) {
+        H += (i128)dig[j] << pos;
+        pos += ctx->bits[j];
+    }
+    const unsigned m = ctx->top_m;
+    const i128 K = (i128)ctx->k;
+    i128 lo = H & (((i128)1 << m) - 1), hi = H >> m;
+    i128 q = hi >= 0 ? hi / K : -((-hi + K - 1) / K);
+    i128 top = lo + ((hi - q * K) << m);
+    if (2 * top >= (K << m)) {
+        top -= K << m;
+        q += 1;
+    }
+    for (uint32_t j = t; j + 1 < L; j++) {
+        unsigned b = ctx->bits[j];
+        i128 qq = (top + ((i128)1 << (b - 1))) >> b;
+        dig[j] = (int64_t)(top - (qq << b));
+        top = qq;
+    }
+    if (t < L) dig[L - 1] = (int64_t)top;
+    return (int64_t)q;
+}
+
+void limbs_to_digits(kite_ctx *ctx, const uint64_t *limbs, int64_t *dig);
+
+}  // namespace kite
+
+using namespace kite;
+
+extern "C" void kite_carry_digits(const kite_ctx *ctx, int64_t *dig) {
+    const uint32_t L = ctx->L;
+    int64_t cy = 0;
+    for (uint32_t j = 0; j < L; j++) cy = split_bal(dig[j] + cy, ctx->bits[j], &dig[j]);
+    for (int pass = 0; pass < 64; pass++) {
+        cy = fold_top(ctx, dig, cy);
+        for (uint32_t j = 0; j < ctx->top_t && cy; j++) cy = split_bal(dig[j] + cy, ctx->bits[j], &dig[j]);
+        if (!cy) return;
+        if (ctx->top_t < L) {
+            dig[ctx->top_t] += cy;
+            cy = 0;
+        }
+    }
+}
+
+extern "C" uint64_t kite_s_at(const kite_ctx *ctx, uint64_t j) {  // src/ctx.c:169-183
+    const u128 D = (u128)ctx->L << 64;
+    return (uint64_t)(((u128)j * tfix(ctx) + D - 1) / D);
+}
+
+extern "C" double kite_weight(const kite_ctx *ctx, uint64_t j, uint64_t sj) {  // src/ctx.c:185-196
+    // 2^(s_j - j T'/L): exponent = (s_j*D - j*T'fix) / D in [0, 1), top 64 fraction bits -> exp2
+    const u128 D = (u128)ctx->L << 64;
+    u128 num = (u128)sj * D - (u128)j * tfix(ctx);
+    double f = (double)(num / ctx->L) / 18446744073709551616.0;
+    return exp2(f);
+}
+
+// ---- FFT length choice (src/ctx.c:26-148) ------------------------------------------------------
+
+extern "C" double kite_safe_bpw(uint32_t L, uint64_t k) {
+    // The reference's fitted model (src/ctx.c:26-33): 21.44 - 0.305 (log2 L - 12) - 0.5 log2 k
+    // less a 0.2 margin. The gfx950 engine's round-off stays below it at every length measured
+    // (tools: `kite sweep`), so the same bound is used.
+    return 21.44 - 0.305 * (std::log2((double)L) - 12) - 0.50 * std::log2((double)k) - 0.2;
+}
+
+namespace {
+struct TuneTable {
+    std::once_flag once;
+    std::vector<uint32_t> L;
+    std::vector<double> us;
+    std::vector<uint8_t> ord;
+} g_tune;
+
+void load_tune() {
+    std::call_once(g_tune.once, [] {
+        const char *p = getenv("KITE_FFT_TIMES");
+        std::string path;
+        if (!p) {
+            const char *home = getenv("HOME");
+            if (!home) return;
+            path = std::string(home) + "/.kite/fft_times_mi355x.txt";
+            p = path.c_str();
+        }
+        FILE *f = fopen(p, "r");
+        if (!f) return;
+        char line[256];
+        while (fgets(line, sizeof line, f)) {
+            unsigned L, ord;
+            double us;
+            int got = sscanf(line, "%u %lf %u", &L, &us, &ord);
+            if (got < 2) continue;
+            if (got < 3) ord = 0;
+            g_tune.L.push_back(L);
+            g_tune.us.push_back(us);
+            g_tune.ord.push_back((uint8_t)ord);
+        }
+        fclose(f);
+    });
+}
+
+double tuned_us(uint32_t L) {
+    for (size_t i = 0; i < g_tune.L.size(); i++)
+        if (g_tune.L[i] == L) return g_tune.us[i];
+    return -1;
+}
+
+double cost_model(uint32_t L) {  // odd radices cost more per point (src/ctx.c:94-103)
+    double c = std::log2((double)L);
+    uint32_t m = L;
+    while (m % 3 == 0) { m /= 3; c += 0.6; }
+    while (m % 5 == 0) { m /= 5; c += 1.0; }
+    while (m % 7 == 0) { m /= 7; c += 1.4; }
+    return (double)L * c;
+}
+
+bool usable(uint32_t L) { return L % 64 == 0 && fft_len_ok(L) && engine_shape_ok(L, 0); }
+}  // namespace
+
+uint32_t kite_tuned_order(uint32_t L) {
+    load_tune();
+    for (size_t i = 0; i < g_tune.L.size(); i++)
+        if (g_tune.L[i] == L) return g_tune.ord[i];
+    return 0;
+}
+
+extern "C" uint32_t kite_choose_fft_len(uint64_t k, uint64_t n) {
+    if (n < 2048) {  // tiny numbers: short power-of-two lengths
+        for (uint32_t L = 64; L <= n; L *= 2)
+            if ((double)n / L <= kite_safe_bpw(L, k) && engine_shape_ok(L, 0)) return L;
+        return 0;
+    }
+    uint32_t lo = 0;
+    for (uint32_t L = 64; L < (1u << 28); L += 64)
+        if (usable(L) && (double)n / L <= kite_safe_bpw(L, k)) {
+            lo = L;
+            break;
+        }
+    if (!lo) return 0;
+    load_tune();
+    bool all = true;
+    for (uint32_t L = lo; L <= lo + lo / 10 * 3; L += 64)
+        if (usable(L) && tuned_us(L) < 0) all = false;
+    uint32_t best = lo;
+    double bc = 1e300;
+    for (uint32_t L = lo; L <= lo + lo / 10 * 3; L += 64) {
+        if (!usable(L)) continue;
+        double c = all ? tuned_us(L) : cost_model(L);
+        if (c < bc) {
+            bc = c;
+            best = L;
+        }
+    }
+    return best;
+}
+
+extern "C" uint32_t kite_next_fft_len(uint32_t L) {
+    for (uint32_t M = (L / 64 + 1) * 64; M < (1u << 28); M += 64)
+        if (fft_len_ok(M) && engine_shape_ok(M, 0)) return M;
+    return 0;
+}
+
+// ---- limbs (src/ctx.c:409-551) ---------------------------------------------------------------
+
+namespace {
+
+void limbs_add_at(uint64_t *a, size_t na, uint64_t pos, uint64_t v) {
+    size_t w = pos / 64;
+    if (w >= na || !v) return;
+    unsigned sh = pos % 64;
+    u128 t = (u128)v << sh;
+    u128 s = (u128)a[w] + (uint64_t)t;
+    a[w] = (uint64_t)s;
+    uint64_t cy = (uint64_t)(s >> 64) + (uint64_t)(t >> 64);
+    for (size_t i = w + 1; cy && i < na; i++) {
+        s = (u128)a[i] + cy;
+        a[i] = (uint64_t)s;
+        cy = (uint64_t)(s >> 64);
+    }
+}
+
+int limbs_cmp(const uint64_t *a, const uint64_t *b, size_t n) {
+    for (size_t i = n; i-- > 0;)
+        if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+
+void limbs_sub(uint64_t *a, const uint64_t *b, size_t n) {
+    uint64_t br = 0;
+    for (size_t i = 0; i < n; i++) {
+        u128 d = (u128)a[i] - b[i] - br;
+        a[i] = (uint64_t)d;
+        br = (uint64_t)(d >> 64) & 1;
+    }
+}
+
+uint64_t limbs_bits(const uint64_t *a, size_t na, uint64_t pos, unsigned cnt) {
+    size_t w = pos / 64;
+    if (w >= na) return 0;
+    u128 v = a[w];
+    if (w + 1 < na) v |= (u128)a[w + 1] << 64;
+    v >>= pos % 64;
+    return cnt >= 64 ? (uint64_t)v : (uint64_t)v & ((1ull << cnt) - 1);
+}
+
+// a mod N with k*2^n == 1: a = lo + h*2^n, h = q*k + r  ->  lo + q + r*2^n; then subtract N.
+void reduce_mod_N(const kite_ctx *ctx, uint64_t *a, size_t na) {
+    const uint64_t n = ctx->n, k = ctx->k;
+    for (int pass = 0; pass < 4; pass++) {
+        size_t w = n / 64;
+        unsigned sh = n % 64;
+        u128 h = 0;
+        for (size_t i = na; i-- > w;) h = (h << 64) | a[i];
+        h >>= sh;
+        if (!h) break;
+        a[w] &= sh ? (1ull << sh) - 1 : 0;
+        for (size_t i = w + 1; i < na; i++) a[i] = 0;
+        u128 q = h / k;
+        limbs_add_at(a, na, n, (uint64_t)(h % k));
+        limbs_add_at(a, na, 0, (uint64_t)q);
+        limbs_add_at(a, na, 64, (uint64_t)(q >> 64));
+    }
+    while (limbs_cmp(a, ctx->Nl.data(), ctx->nlimbs) >= 0) limbs_sub(a, ctx->Nl.data(), ctx->nlimbs);
+}
+
+// k * (sum of the positive, or of the negated negative, digits) mod N
+void signed_part(const kite_ctx *ctx, const int64_t *dig, int sign, uint64_t *a, size_t na) {
+    memset(a, 0, na * 8);
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < ctx->L; j++) {
+        int64_t d = dig[j];
+        if (sign > 0 && d > 0) limbs_add_at(a, na, s, (uint64_t)d);
+        if (sign < 0 && d < 0) limbs_add_at(a, na, s, (uint64_t)(-d));
+        s += ctx->bits[j];
+    }
+    u128 cy = 0;
+    for (size_t i = 0; i < na; i++) {
+        cy += (u128)a[i] * ctx->k;
+        a[i] = (uint64_t)cy;
+        cy >>= 64;
+    }
+    reduce_mod_N(ctx, a, na);
+}
+
+}  // namespace
+
+namespace kite {
+
+void digits_to_limbs(kite_ctx *ctx, const int64_t *dig, uint64_t *limbs) {
+    size_t na = (ctx->sL + 64) / 64 + 3;
+    std::vector<uint64_t> p(na), m(na);
+    signed_part(ctx, dig, 1, p.data(), na);
+    signed_part(ctx, dig, -1, m.data(), na);
+    if (limbs_cmp(p.data(), m.data(), ctx->nlimbs) < 0) {
+        uint64_t cy = 0;
+        for (size_t i = 0; i < ctx->nlimbs; i++) {
+            u128 t = (u128)p[i] + ctx->Nl[i] + cy;
+            p[i] = (uint64_t)t;
+            cy = (uint64_t)(t >> 64);
+        }
+    }
+    limbs_sub(p.data(), m.data(), ctx->nlimbs);
+    memcpy(limbs, p.data(), ctx->nlimbs * 8);
+}
+
+void limbs_to_digits(kite_ctx *ctx, const uint64_t *limbs, int64_t *dig) {
+    size_t nl = ctx->nlimbs + 1;
+    std::vector<uint64_t> a(nl);
+    memcpy(a.data(), limbs, ctx->nlimbs * 8);
+    // V = x*2^n mod N: x = q*k + r -> q + r*2^n
+    u128 r = 0;
+    for (size_t i = nl; i-- > 0;) {
+        r = (r << 64) | a[i];
+        a[i] = (uint64_t)(r / ctx->k);
+        r %= ctx->k;
+    }
+    limbs_add_at(a.data(), nl, ctx->n, (uint64_t)r);
+    if (limbs_cmp(a.data(), ctx->Nl.data(), ctx->nlimbs) >= 0 && !a[nl - 1])
+        limbs_sub(a.data(), ctx->Nl.data(), ctx->nlimbs);
+    uint64_t s = 0;
+    for (uint32_t j = 0; j + 1 < ctx->L; j++) {
+        dig[j] = (int64_t)limbs_bits(a.data(), nl, s, ctx->bits[j]);
+        s += ctx->bits[j];
+    }
+    dig[ctx->L - 1] = (int64_t)limbs_bits(a.data(), nl, s, 64);
+    kite_carry_digits(ctx, dig);
+}
+
+}  // namespace kite
+
+// ---- base^(N+1) -> base^(N-1), CRT when gcd(base, N) > 1 (src/prp.c:36-144) ------------------
+
+namespace {
+uint64_t mod_small(const uint64_t *a, size_t n, uint64_t m) {
+    u128 r = 0;
+    for (size_t i = n; i-- > 0;) r = ((r << 64) | a[i]) % m;
+    return (uint64_t)r;
+}
+
+void divexact(uint64_t *a, size_t n, uint64_t d, u128 top) {
+    u128 r = top;
+    for (size_t i = n; i-- > 0;) {
+        r = (r << 64) | a[i];
+        a[i] = (uint64_t)(r / d);
+        r %= d;
+    }
+}
+
+// a <- a / d (mod M), gcd(d, M) = 1
+void div_mod(uint64_t *a, const uint64_t *M, size_t n, uint64_t d) {
+    uint64_t am = mod_small(a, n, d), mm = mod_small(M, n, d), t = 0;
+    while ((am + (u128)t * mm) % d) t++;
+    u128 cy = 0;
+    for (size_t i = 0; i < n; i++) {
+        cy += (u128)a[i] + (u128)M[i] * t;
+        a[i] = (uint64_t)cy;
+        cy >>= 64;
+    }
+    divexact(a, n, d, cy);
+}
+}  // namespace
+
+extern "C" int kite_residue_nm1(uint64_t *y, const uint64_t *N, size_t n, uint64_t base) {
+    uint64_t Q = 1, b = base;
+    std::vector<uint64_t> M(n + 1), t(n + 1);
+    memcpy(M.data(), N, n * 8);
+    for (uint64_t p = 2; b > 1; p++) {
+        if (b % p) continue;
+        while (b % p == 0) b /= p;
+        while (mod_small(M.data(), n, p) == 0) {
+            if (Q > (1ull << 40)) return -1;
+            Q *= p;
+            divexact(M.data(), n, p, 0);
+        }
+    }
+    if (Q == 1) {
+        div_mod(y, N, n, base);
+        div_mod(y, N, n, base);
+        return 0;
+    }
+    // y mod M = (y*Q mod N) / Q
+    u128 cy = 0;
+    for (size_t i = 0; i < n; i++) {
+        cy += (u128)y[i] * Q;
+        t[i] = (uint64_t)cy;
+        cy >>= 64;
+    }
+    t[n] = (uint64_t)cy;
+    for (;;) {
+        bool ge = t[n] != 0 || limbs_cmp(t.data(), N, n) >= 0;
+        if (!ge) break;
+        uint64_t br = 0;
+        for (size_t i = 0; i <= n; i++) {
+            u128 d = (u128)t[i] - (i < n ? N[i] : 0) - br;
+            t[i] = (uint64_t)d;
+            br = (uint64_t)(d >> 64) & 1;
+        }
+    }
+    divexact(t.data(), n + 1, Q, 0);
+    div_mod(t.data(), M.data(), n, base);
+    div_mod(t.data(), M.data(), n, base);
+    uint64_t rm = mod_small(t.data(), n, Q), mm = mod_small(M.data(), n, Q), u = 0;
+    while ((rm + (u128)mm * u) % Q) u++;
+    cy = 0;
+    for (size_t i = 0; i < n; i++) {
+        cy += (u128)t[i] + (u128)M[i] * u;
+        y[i] = (uint64_t)cy;
+        cy >>= 64;
+    }
+    return 0;
+}

@@ -1,0 +1,83 @@
+#!/usr/bin/env python3
+"""Writes the GMP golden fixtures under tests/golden/ (run in the build container, not on a GPU box).
+
+  python tests/golden/make_golden.py plain_small     # 16 seeded n~65K tests: RES64 by GMP
+  python tests/golden/make_golden.py chain           # x_i = 3^(k*2^i) mod N at chosen i
+  python tests/golden/make_golden.py batch_subset N  # first N tests of the 256-test batch
+  python tests/golden/make_golden.py single_large    # the seeded n~660K test
+
+GMP (mpz_powm, the semantics of the reference's tools/gmp_ref.c) runs through gmp_res64.c,
+built here against the system libgmp. Outputs are CSV lines "k,n,base,RES64" (or
+"k,n,base,i,low64,fnv64" for chain residues, fnv64 = FNV-1a-64 over the little-endian limbs,
+(n+127)/64 limbs).
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from ahnum_amd import workloads  # noqa: E402
+
+BIN = os.path.join(HERE, "_bin", "gmp_res64")
+
+
+def build():
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    lib = "/usr/lib/x86_64-linux-gnu/libgmp.so.10"
+    if not os.path.exists(lib):
+        lib = "/opt/conda/lib/libgmp.so"
+    subprocess.check_call(["gcc", "-O2", "-I/opt/conda/include", os.path.join(HERE, "gmp_res64.c"),
+                           "-o", BIN, lib])
+
+
+def gmp_lines(lines, jobs):
+    def one(line):
+        return subprocess.run(["nice", "-n", "15", BIN], input=line + "\n", capture_output=True,
+                              text=True, check=True).stdout.strip()
+    with ThreadPoolExecutor(jobs) as ex:
+        return list(ex.map(one, lines))
+
+
+def write(name, header, rows):
+    path = os.path.join(HERE, name)
+    with open(path, "w") as f:
+        f.write(header + "\n")
+        for r in rows:
+            f.write(r + "\n")
+    print("wrote", path, len(rows), "rows")
+
+
+def main():
+    what = sys.argv[1]
+    jobs = int(os.environ.get("GOLDEN_JOBS", "4"))
+    build()
+    if what == "plain_small":
+        tests = workloads.plain_small()
+        rows = gmp_lines([f"{k} {n} 3" for k, n in tests], jobs)
+        write("res64_plain_small.csv", "# k,n,base,RES64 (GMP mpz_powm): workloads.plain_small(16, seed=1)", rows)
+    elif what == "chain":
+        cases = []
+        for k, n in workloads.plain_small()[:4]:
+            for i in (0, 1, 7, 100, 1000):
+                cases.append(f"{k} {n} 3 x {i}")
+        for k, n in workloads.single_large() + workloads.batch_mixed()[:2]:
+            for i in (0, 3, 50):
+                cases.append(f"{k} {n} 3 x {i}")
+        rows = gmp_lines(cases, jobs)
+        write("chain_x.csv", "# k,n,base,i,low64,fnv64 of x_i = base^(k*2^i) mod N (GMP)", rows)
+    elif what == "batch_subset":
+        cnt = int(sys.argv[2])
+        tests = workloads.batch_mixed()[:cnt]
+        rows = gmp_lines([f"{k} {n} 3" for k, n in tests], jobs)
+        write("res64_batch_subset.csv", f"# k,n,base,RES64 (GMP): first {cnt} of workloads.batch_mixed(256, seed=3)", rows)
+    elif what == "single_large":
+        rows = gmp_lines([f"{k} {n} 3" for k, n in workloads.single_large()], 1)
+        write("res64_single_large.csv", "# k,n,base,RES64 (GMP): workloads.single_large(seed=2)", rows)
+    else:
+        raise SystemExit("unknown fixture set " + what)
+
+
+if __name__ == "__main__":
+    main()
