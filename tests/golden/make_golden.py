@@ -32,10 +32,15 @@ def build():
                            "-o", BIN, lib])
 
 
-def gmp_lines(lines, jobs):
+def gmp_lines(lines, jobs, partial=None):
+    """Runs GMP on each input line; with `partial`, appends each result there as it completes."""
     def one(line):
-        return subprocess.run(["nice", "-n", "15", BIN], input=line + "\n", capture_output=True,
-                              text=True, check=True).stdout.strip()
+        out = subprocess.run(["nice", "-n", "15", BIN], input=line + "\n", capture_output=True,
+                             text=True, check=True).stdout.strip()
+        if partial:
+            with open(partial, "a") as f:
+                f.write(out + "\n")
+        return out
     with ThreadPoolExecutor(jobs) as ex:
         return list(ex.map(one, lines))
 
@@ -70,10 +75,12 @@ def main():
     elif what == "batch_subset":
         cnt = int(sys.argv[2])
         tests = workloads.batch_mixed()[:cnt]
-        rows = gmp_lines([f"{k} {n} 3" for k, n in tests], jobs)
+        rows = gmp_lines([f"{k} {n} 3" for k, n in tests], jobs,
+                         partial=os.path.join(HERE, "_bin", "batch_subset.partial"))
         write("res64_batch_subset.csv", f"# k,n,base,RES64 (GMP): first {cnt} of workloads.batch_mixed(256, seed=3)", rows)
     elif what == "single_large":
-        rows = gmp_lines([f"{k} {n} 3" for k, n in workloads.single_large()], 1)
+        rows = gmp_lines([f"{k} {n} 3" for k, n in workloads.single_large()], 1,
+                         partial=os.path.join(HERE, "_bin", "single_large.partial"))
         write("res64_single_large.csv", "# k,n,base,RES64 (GMP): workloads.single_large(seed=2)", rows)
     else:
         raise SystemExit("unknown fixture set " + what)
