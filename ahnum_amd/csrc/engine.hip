@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "engine_dev.h"
+#include "kite_batch.h"
 #include "kite_internal.h"
 
 typedef unsigned __int128 u128;
@@ -288,10 +289,11 @@ __device__ __forceinline__ void row_pair(int R, int u, int &ra, int &rb) {
 // ---- kernels -----------------------------------------------------------------------------------
 
 extern "C" __global__ void __launch_bounds__(kNT) k_rows(const SlotDev *slots, SlotDev one,
-                                                       const int2 *jobs, int mode) {
+                                                      const int2 *jobs, int mode, int step) {
     extern __shared__ double2 lds[];
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
     const ShapeDev &sh = *s.shp;
     const int R = sh.R, C = sh.C;
     int ra, rb;
@@ -396,10 +398,11 @@ __device__ __forceinline__ double pow2i(int e) { return e == 0 ? 1.0 : e == 1 ? 
 __device__ __forceinline__ double pow2ni(int e) { return e == 0 ? 1.0 : e == 1 ? 0.5 : 0.25; }
 
 extern "C" __global__ void __launch_bounds__(kNT) k_cols_inv(const SlotDev *slots, SlotDev one,
-                                                           const int2 *jobs, int mode) {
+                                                      const int2 *jobs, int mode, int step) {
     extern __shared__ double2 lds[];
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
     const ShapeDev &sh = *s.shp;
     const int R = sh.R, C = sh.C, G = sh.G, NG = sh.NG, g = jb.y, c0 = g * G;
     double2 *A = lds, *B = lds + R * G;
@@ -454,10 +457,11 @@ extern "C" __global__ void __launch_bounds__(kNT) k_cols_inv(const SlotDev *slot
 }
 
 extern "C" __global__ void __launch_bounds__(kNT) k_cols_fwd(const SlotDev *slots, SlotDev one,
-                                                           const int2 *jobs, int mode) {
+                                                      const int2 *jobs, int mode, int step) {
     extern __shared__ double2 lds[];
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
     const ShapeDev &sh = *s.shp;
     const int R = sh.R, C = sh.C, G = sh.G, NG = sh.NG, g = jb.y, c0 = g * G;
     double2 *A = lds, *B = lds + R * G;
@@ -798,6 +802,7 @@ int engine_init(kite_ctx *ctx) {
     s.cout = E->cout;
     s.yfreq = E->yfreq;
     s.maxerr = E->maxerr;
+    s.count = 1;
     if (fill_slot_fold(ctx, sh, &s)) {
         g_err = "top region does not fit the last chunk";
         return -1;
@@ -844,16 +849,16 @@ static void launch_rows(Engine *E, double2 *x, const double2 *ysrc, int mode) {
     s.x = x;
     s.y_src = ysrc;
     hipLaunchKernelGGL(k_rows, dim3(E->nrowjobs), dim3(kNT), E->lds_rows, E->stream, (const SlotDev *)nullptr, s,
-                       (const int2 *)nullptr, mode);
+                       (const int2 *)nullptr, mode, 0);
 }
 
 static void launch_cols(Engine *E, double2 *x, int mode) {
     SlotDev s = E->slot;
     s.x = x;
     hipLaunchKernelGGL(k_cols_inv, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, mode);
+                       s, (const int2 *)nullptr, mode, 0);
     hipLaunchKernelGGL(k_cols_fwd, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, mode);
+                       s, (const int2 *)nullptr, mode, 0);
 }
 
 void engine_square(kite_ctx *ctx, void *x, uint64_t count) {
@@ -891,7 +896,7 @@ void engine_from_digits(kite_ctx *ctx, void *x, const int64_t *dig) {
     SlotDev s = E->slot;
     s.x = (double2 *)x;
     hipLaunchKernelGGL(k_cols_fwd, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, (int)kModeDigits);
+                       s, (const int2 *)nullptr, (int)kModeDigits, 0);
     hipStreamSynchronize(E->stream);  // hdig is reused
 }
 
@@ -903,7 +908,7 @@ void engine_to_digits(kite_ctx *ctx, const void *x, int64_t *dig) {
     // the conversion's own rounding error is not an operation's round-off: count it aside
     s.maxerr = E->maxerr + 1;
     hipLaunchKernelGGL(k_cols_inv, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, (int)kModeDigits);
+                       s, (const int2 *)nullptr, (int)kModeDigits, 0);
     hipMemcpyAsync(E->hdig.data(), E->dig, (size_t)ctx->L * 4, hipMemcpyDeviceToHost, E->stream);
     hipStreamSynchronize(E->stream);
     const uint32_t R = sh.R, C = sh.C, G = sh.G, NG = sh.NG;
@@ -929,3 +934,288 @@ void engine_reset_maxerr(kite_ctx *ctx) {
 void engine_sync(kite_ctx *ctx) { hipStreamSynchronize(ctx->eng->stream); }
 
 }  // namespace kite
+
+// ================================================================================================
+// Batch: many contexts advanced by shared launches (include/kite_batch.h)
+// ================================================================================================
+
+namespace kite {
+
+struct BatchSlot {
+    kite_ctx *ctx = nullptr;
+    std::vector<kite_num *> nums;
+};
+
+}  // namespace kite
+
+struct kite_batch {
+    std::vector<kite::BatchSlot> slots;
+    hipStream_t stream = nullptr;
+    kite::SlotDev *d_slots = nullptr, *h_slots = nullptr;  // h_slots is pinned
+    size_t slot_cap = 0;
+    int2 *d_rows = nullptr, *d_cols = nullptr;
+    int nrows = 0, ncols = 0;
+    size_t jobs_cap = 0;
+    size_t lds_rows = 0, lds_cols = 0;
+    unsigned long long *d_maxerr = nullptr;
+    bool dirty = true;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double ms[3] = {0, 0, 0};
+    uint64_t launches[3] = {0, 0, 0};
+};
+
+using namespace kite;
+
+static int batch_refresh(kite_batch *b) {
+    if (!b->dirty) return 0;
+    const size_t cap = b->slots.size();
+    if (cap > b->slot_cap) {
+        const size_t nc = std::max<size_t>(cap, 2 * b->slot_cap);
+        if (b->d_slots) hipFree(b->d_slots);
+        if (b->h_slots) hipHostFree(b->h_slots);
+        unsigned long long *old = b->d_maxerr;
+        HIP_OK(hipMalloc(&b->d_slots, nc * sizeof(SlotDev)));
+        HIP_OK(hipHostMalloc((void **)&b->h_slots, nc * sizeof(SlotDev), hipHostMallocDefault));
+        HIP_OK(hipMalloc(&b->d_maxerr, nc * 8));
+        HIP_OK(hipMemsetAsync(b->d_maxerr, 0, nc * 8, b->stream));
+        if (old) {
+            HIP_OK(hipMemcpyAsync(b->d_maxerr, old, b->slot_cap * 8, hipMemcpyDeviceToDevice, b->stream));
+            HIP_OK(hipStreamSynchronize(b->stream));
+            hipFree(old);
+        }
+        b->slot_cap = nc;
+    }
+    std::vector<int2> rows, cols;
+    size_t lr = 0, lc = 0;
+    for (size_t s = 0; s < cap; s++) {
+        kite_ctx *ctx = b->slots[s].ctx;
+        if (!ctx) continue;
+        Engine *E = ctx->eng;
+        for (int u = 0; u < E->nrowjobs; u++) rows.push_back(make_int2((int)s, u));
+        for (int u = 0; u < E->ncoljobs; u++) cols.push_back(make_int2((int)s, u));
+        lr = std::max(lr, E->lds_rows);
+        lc = std::max(lc, E->lds_cols);
+    }
+    const size_t need = std::max(rows.size(), cols.size());
+    if (need > b->jobs_cap) {
+        if (b->d_rows) hipFree(b->d_rows);
+        if (b->d_cols) hipFree(b->d_cols);
+        const size_t nc = std::max(need, 2 * b->jobs_cap);
+        HIP_OK(hipMalloc(&b->d_rows, nc * sizeof(int2)));
+        HIP_OK(hipMalloc(&b->d_cols, nc * sizeof(int2)));
+        b->jobs_cap = nc;
+    }
+    if (!rows.empty()) HIP_OK(hipMemcpy(b->d_rows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!cols.empty()) HIP_OK(hipMemcpy(b->d_cols, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
+    b->nrows = (int)rows.size();
+    b->ncols = (int)cols.size();
+    b->lds_rows = lr;
+    b->lds_cols = lc;
+    b->dirty = false;
+    return 0;
+}
+
+// Fills the slot table: residue `x` (and operand `y`) per slot, step count per slot (0 = idle).
+static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, const uint32_t *count) {
+    if (batch_refresh(b)) return -1;
+    HIP_OK(hipStreamSynchronize(b->stream));  // the previous table may still be read
+    for (size_t s = 0; s < b->slots.size(); s++) {
+        SlotDev &d = b->h_slots[s];
+        kite_ctx *ctx = b->slots[s].ctx;
+        if (!ctx) {
+            memset(&d, 0, sizeof d);
+            continue;
+        }
+        d = ctx->eng->slot;
+        d.maxerr = b->d_maxerr + s;
+        const bool on = xi[s] >= 0 && (size_t)xi[s] < b->slots[s].nums.size() && count[s] > 0;
+        d.count = on ? count[s] : 0;
+        d.x = on ? (double2 *)b->slots[s].nums[xi[s]]->dev : nullptr;
+        d.y_src = (on && yi && yi[s] >= 0) ? (const double2 *)b->slots[s].nums[yi[s]]->dev : nullptr;
+    }
+    HIP_OK(hipMemcpyAsync(b->d_slots, b->h_slots, b->slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice,
+                          b->stream));
+    return 0;
+}
+
+static void batch_launch(kite_batch *b, int pass, int mode, int step) {
+    const bool rows = pass == 0;
+    const int grid = rows ? b->nrows : b->ncols;
+    if (!grid) return;
+    SlotDev none{};
+    const size_t lds = rows ? b->lds_rows : b->lds_cols;
+    if (pass == 0)
+        hipLaunchKernelGGL(k_rows, dim3(grid), dim3(kNT), lds, b->stream, b->d_slots, none, b->d_rows, mode, step);
+    else if (pass == 1)
+        hipLaunchKernelGGL(k_cols_inv, dim3(grid), dim3(kNT), lds, b->stream, b->d_slots, none, b->d_cols, mode, step);
+    else
+        hipLaunchKernelGGL(k_cols_fwd, dim3(grid), dim3(kNT), lds, b->stream, b->d_slots, none, b->d_cols, mode, step);
+}
+
+extern "C" kite_batch *kite_batch_new(void) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return nullptr;
+    set_lds_limits();
+    kite_batch *b = new kite_batch();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+extern "C" void kite_batch_free(kite_batch *b) {
+    if (!b) return;
+    hipStreamSynchronize(b->stream);
+    for (size_t s = 0; s < b->slots.size(); s++) kite_batch_remove(b, (int)s);
+    for (hipEvent_t e : b->ev) hipEventDestroy(e);
+    if (b->d_slots) hipFree(b->d_slots);
+    if (b->h_slots) hipHostFree(b->h_slots);
+    if (b->d_rows) hipFree(b->d_rows);
+    if (b->d_cols) hipFree(b->d_cols);
+    if (b->d_maxerr) hipFree(b->d_maxerr);
+    hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t fft_len, uint32_t nnums) {
+    kite_opts o{};
+    o.backend = KITE_BACKEND_HIP;
+    o.fft_len = fft_len;
+    kite_ctx *ctx = kite_ctx_new(k, n, -1, &o);
+    if (!ctx) return -1;
+    BatchSlot bs;
+    bs.ctx = ctx;
+    for (uint32_t i = 0; i < nnums; i++) {
+        kite_num *x = kite_num_new(ctx);
+        if (!x) {
+            for (kite_num *y : bs.nums) kite_num_free(ctx, y);
+            kite_ctx_free(ctx);
+            return -1;
+        }
+        bs.nums.push_back(x);
+    }
+    hipStreamSynchronize(ctx->eng->stream);
+    size_t s = 0;
+    while (s < b->slots.size() && b->slots[s].ctx) s++;
+    if (s == b->slots.size()) b->slots.emplace_back();
+    b->slots[s] = std::move(bs);
+    b->dirty = true;
+    if (batch_refresh(b)) return -1;
+    // clear this slot's round-off accumulator
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, b->stream, b->d_maxerr + s, 0ull, 1);
+    hipStreamSynchronize(b->stream);
+    return (int)s;
+}
+
+extern "C" void kite_batch_remove(kite_batch *b, int slot) {
+    if (slot < 0 || (size_t)slot >= b->slots.size() || !b->slots[slot].ctx) return;
+    hipStreamSynchronize(b->stream);
+    BatchSlot &bs = b->slots[slot];
+    for (kite_num *x : bs.nums) kite_num_free(bs.ctx, x);
+    kite_ctx_free(bs.ctx);
+    bs = BatchSlot();
+    b->dirty = true;
+}
+
+extern "C" int kite_batch_capacity(const kite_batch *b) { return (int)b->slots.size(); }
+
+extern "C" kite_ctx *kite_batch_ctx(kite_batch *b, int slot) {
+    return slot >= 0 && (size_t)slot < b->slots.size() ? b->slots[slot].ctx : nullptr;
+}
+
+extern "C" kite_num *kite_batch_num(kite_batch *b, int slot, uint32_t i) {
+    if (slot < 0 || (size_t)slot >= b->slots.size() || i >= b->slots[slot].nums.size()) return nullptr;
+    return b->slots[slot].nums[i];
+}
+
+extern "C" int kite_batch_sync(kite_batch *b) { return hipStreamSynchronize(b->stream) == hipSuccess ? 0 : -1; }
+
+static void timed_launch(kite_batch *b, int pass, int mode, int step, size_t evi) {
+    if (b->timing) hipEventRecord(b->ev[evi], b->stream);
+    batch_launch(b, pass, mode, step);
+    if (b->timing) hipEventRecord(b->ev[evi + 1], b->stream);
+}
+
+extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint32_t *count) {
+    uint32_t maxc = 0;
+    for (size_t s = 0; s < b->slots.size(); s++)
+        if (b->slots[s].ctx && which[s] >= 0) maxc = std::max(maxc, count[s]);
+    if (!maxc) return 0;
+    if (batch_upload(b, which, nullptr, count)) return -1;
+    if (b->timing && b->ev.size() < 6 * (size_t)maxc) {
+        size_t old = b->ev.size();
+        b->ev.resize(6 * (size_t)maxc);
+        for (size_t i = old; i < b->ev.size(); i++) hipEventCreate(&b->ev[i]);
+    }
+    for (uint32_t st = 0; st < maxc; st++) {
+        timed_launch(b, 0, kModeSquare, (int)st, 6 * st);
+        timed_launch(b, 1, kModeSquare, (int)st, 6 * st + 2);
+        timed_launch(b, 2, kModeSquare, (int)st, 6 * st + 4);
+    }
+    if (b->timing) {
+        hipStreamSynchronize(b->stream);
+        for (int p = 0; p < 3; p++) b->ms[p] = 0;
+        for (uint32_t st = 0; st < maxc; st++)
+            for (int p = 0; p < 3; p++) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, b->ev[6 * st + 2 * p], b->ev[6 * st + 2 * p + 1]);
+                b->ms[p] += ms;
+            }
+        for (int p = 0; p < 3; p++) b->launches[p] = maxc;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int kite_batch_mul(kite_batch *b, const int32_t *dst, const int32_t *src) {
+    std::vector<uint32_t> one(b->slots.size(), 1);
+    bool any = false;
+    for (size_t s = 0; s < b->slots.size(); s++) any |= b->slots[s].ctx && dst[s] >= 0;
+    if (!any) return 0;
+    if (batch_upload(b, dst, src, one.data())) return -1;
+    batch_launch(b, 0, kModeRowFft, 0);
+    batch_launch(b, 0, kModeMul, 0);
+    batch_launch(b, 1, kModeMul, 0);
+    batch_launch(b, 2, kModeMul, 0);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int kite_batch_copy(kite_batch *b, const int32_t *dst, const int32_t *src) {
+    for (size_t s = 0; s < b->slots.size(); s++) {
+        BatchSlot &bs = b->slots[s];
+        if (!bs.ctx || dst[s] < 0 || dst[s] == src[s]) continue;
+        hipMemcpyAsync(bs.nums[dst[s]]->dev, bs.nums[src[s]]->dev, (size_t)bs.ctx->eng->H * sizeof(double2),
+                       hipMemcpyDeviceToDevice, b->stream);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
+    if (batch_refresh(b)) return -1;
+    const size_t cap = b->slots.size();
+    if (!cap) return 0;
+    std::vector<unsigned long long> v(cap);
+    HIP_OK(hipMemcpyAsync(v.data(), b->d_maxerr, cap * 8, hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    for (size_t s = 0; s < cap; s++) memcpy(&out[s], &v[s], 8);
+    if (reset) {
+        hipLaunchKernelGGL(k_set_u64, dim3((unsigned)((cap + 63) / 64)), dim3(64), 0, b->stream, b->d_maxerr, 0ull,
+                           (int)cap);
+        HIP_OK(hipStreamSynchronize(b->stream));
+    }
+    return 0;
+}
+
+extern "C" int kite_batch_set_timing(kite_batch *b, int on) {
+    b->timing = on != 0;
+    return 0;
+}
+
+extern "C" int kite_batch_pass_times(kite_batch *b, double *out3, uint64_t *launches3) {
+    for (int p = 0; p < 3; p++) {
+        out3[p] = b->ms[p];
+        if (launches3) launches3[p] = b->launches[p];
+    }
+    return 0;
+}

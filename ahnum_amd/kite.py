@@ -286,3 +286,134 @@ def llr(k, n, fft_len=0, **hooks):
     r = LlrResult()
     rc = lib().kite_llr(k, n, ctypes.byref(o), ctypes.byref(r))
     return rc, r
+
+
+# ---- batched engine (include/kite_batch.h) -----------------------------------------------------
+
+class BatchTest(ctypes.Structure):
+    _fields_ = [("k", c_u64), ("n", c_u64), ("base", c_u32), ("fft_len", c_u32)]
+
+
+DONE_FN = ctypes.CFUNCTYPE(None, c_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(PrpResult))
+
+
+class BatchPrpOpts(ctypes.Structure):
+    _fields_ = [
+        ("slots", c_u32),
+        ("gerbicz_block", c_u32),
+        ("gerbicz_blocks", c_u32),
+        ("max_roundoff", c_dbl),
+        ("inject_fault_test", c_u64),
+        ("inject_fault_at", c_u64),
+        ("on_done", DONE_FN),
+        ("arg", c_p),
+    ]
+
+
+I32P = ctypes.POINTER(ctypes.c_int32)
+U32P = ctypes.POINTER(c_u32)
+SIGNATURES.update({
+    "kite_batch_new": (c_p, []),
+    "kite_batch_free": (None, [c_p]),
+    "kite_batch_add": (ctypes.c_int, [c_p, c_u64, c_u64, c_u32, c_u32]),
+    "kite_batch_remove": (None, [c_p, ctypes.c_int]),
+    "kite_batch_capacity": (ctypes.c_int, [c_p]),
+    "kite_batch_ctx": (c_p, [c_p, ctypes.c_int]),
+    "kite_batch_num": (c_p, [c_p, ctypes.c_int, c_u32]),
+    "kite_batch_square": (ctypes.c_int, [c_p, I32P, U32P]),
+    "kite_batch_mul": (ctypes.c_int, [c_p, I32P, I32P]),
+    "kite_batch_copy": (ctypes.c_int, [c_p, I32P, I32P]),
+    "kite_batch_maxerr": (ctypes.c_int, [c_p, ctypes.POINTER(c_dbl), ctypes.c_int]),
+    "kite_batch_sync": (ctypes.c_int, [c_p]),
+    "kite_batch_set_timing": (ctypes.c_int, [c_p, ctypes.c_int]),
+    "kite_batch_pass_times": (ctypes.c_int, [c_p, ctypes.POINTER(c_dbl), ctypes.POINTER(c_u64)]),
+    "kite_batch_prp": (ctypes.c_int, [ctypes.POINTER(BatchTest), ctypes.c_size_t, ctypes.POINTER(BatchPrpOpts),
+                                      ctypes.POINTER(PrpResult), ctypes.POINTER(ctypes.c_int)]),
+})
+
+
+def batch_prp(tests, slots=0, block=1000, blocks=0, fault_test=0, fault_at=0, on_done=None):
+    """Runs Fermat base-3 tests [(k, n), ...] through the batched engine. Returns [(status, PrpResult)]."""
+    arr = (BatchTest * len(tests))(*[BatchTest(k, n, 3, 0) for k, n in tests])
+    o = BatchPrpOpts()
+    o.slots = slots
+    o.gerbicz_block = block
+    o.gerbicz_blocks = blocks
+    o.inject_fault_test = fault_test
+    o.inject_fault_at = fault_at
+    keep = None
+    if on_done is not None:
+        keep = DONE_FN(lambda a, i, st, r: on_done(i, st, r.contents))
+        o.on_done = keep
+    res = (PrpResult * len(tests))()
+    st = (ctypes.c_int * len(tests))()
+    rc = lib().kite_batch_prp(arr, len(tests), ctypes.byref(o), res, st)
+    return rc, [(st[i], res[i]) for i in range(len(tests))]
+
+
+class Batch:
+    """A kite_batch: slots of independent residues advanced by shared launches."""
+
+    def __init__(self):
+        self.lib = lib()
+        self.p = self.lib.kite_batch_new()
+        if not self.p:
+            raise RuntimeError("kite_batch_new failed (no GPU?)")
+        self.slots = {}
+
+    def add(self, k, n, fft_len=0, nnums=1):
+        s = self.lib.kite_batch_add(self.p, k, n, fft_len, nnums)
+        if s < 0:
+            raise RuntimeError(f"kite_batch_add failed for {k}*2^{n}-1")
+        self.slots[s] = (k, n)
+        return s
+
+    def ctx(self, s):
+        return self.lib.kite_batch_ctx(self.p, s)
+
+    def set(self, s, i, v):
+        c = self.ctx(s)
+        nl = self.lib.kite_limbs(c)
+        self.lib.kite_batch_sync(self.p)
+        self.lib.kite_set_limbs(c, self.lib.kite_batch_num(self.p, s, i), to_limbs(v, nl))
+
+    def get(self, s, i):
+        c = self.ctx(s)
+        nl = self.lib.kite_limbs(c)
+        self.lib.kite_batch_sync(self.p)
+        arr = (c_u64 * nl)()
+        self.lib.kite_get_limbs(c, self.lib.kite_batch_num(self.p, s, i), arr)
+        return from_limbs(arr)
+
+    def fft_len(self, s):
+        return self.lib.kite_fft_len(self.ctx(s))
+
+    def square(self, count, which=0):
+        cap = self.lib.kite_batch_capacity(self.p)
+        w = (ctypes.c_int32 * cap)(*[which if s in self.slots else -1 for s in range(cap)])
+        c = (c_u32 * cap)(*[count] * cap)
+        if self.lib.kite_batch_square(self.p, w, c):
+            raise RuntimeError("kite_batch_square failed")
+
+    def maxerr(self, reset=False):
+        cap = self.lib.kite_batch_capacity(self.p)
+        out = (c_dbl * cap)()
+        self.lib.kite_batch_maxerr(self.p, out, int(reset))
+        return {s: out[s] for s in self.slots}
+
+    def sync(self):
+        self.lib.kite_batch_sync(self.p)
+
+    def timing(self, on):
+        self.lib.kite_batch_set_timing(self.p, int(on))
+
+    def pass_times(self):
+        ms = (c_dbl * 3)()
+        n = (c_u64 * 3)()
+        self.lib.kite_batch_pass_times(self.p, ms, n)
+        return list(ms), list(n)
+
+    def close(self):
+        if self.p:
+            self.lib.kite_batch_free(self.p)
+            self.p = None

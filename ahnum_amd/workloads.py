@@ -50,9 +50,9 @@ def plain_small(count=16, seed=1):
     return [(g.odd(1001, 9999), g.uniform_int(60000, 70000)) for _ in range(count)]
 
 
-def single_large(seed=2):
+def single_large(count=1, seed=2):
     g = XorShift64(seed)
-    return [(g.odd(1001, 9999), g.uniform_int(650000, 670000))]
+    return [(g.odd(1001, 9999), g.uniform_int(650000, 670000)) for _ in range(count)]
 
 
 def batch_mixed(count=256, seed=3):

@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X squaring engine on the Fermat workload at n ~ 660K (FFT ~49152).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--cpu-secs S] [--no-cpu]
+
+A step is one squaring of every residue in the batch (B Fermat base-3 tests with k odd in
+[1001, 9999], n in [650000, 670000], seeded): the batched engine advances all B tests with one
+launch per pass. The same number is also timed alone through the kite.h path (the plain
+one-test-at-a-time GPU path). tests/hour follows the reference's throughput formula
+(tools/kite.c:262): sum over tests of squarings/s * 3600 / n.
+
+Prints one JSON line. `roofline` is for the dominant pass kernel, timed with HIP events on the
+batch stream over the timed region; algorithmic bytes per launch are 16 L per test for the row
+pass (read + write of the 8L-byte residue) and 8 L per test for each column pass (32 L per
+squaring in total). `cpu_baseline` is the oracle's restatement of the reference's portable
+two-pass backend (--backend c2p) timed on the host cores, one test per core.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def cpu_worker(args):
+    k, n, secs = args
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    from ahnum_amd import workloads
+    c = O.OCtx(k, n, 0, O.OCtx.C2P)
+    x = c.num(workloads.random_residue_limbs(c.N, 2))
+    O.lib().or_square(c.p, x)
+    t0 = time.time()
+    it = 0
+    while time.time() - t0 < secs:
+        O.lib().or_square(c.p, x)
+        it += 1
+    dt = time.time() - t0
+    L = c.L
+    c.close()
+    return it / dt, L
+
+
+def cpu_baseline(tests, secs):
+    cores = min(os.cpu_count() or 1, 16)
+    jobs = [(k, n, secs) for k, n in (tests * cores)[:cores]]
+    with mp.get_context("spawn").Pool(cores) as pool:
+        out = pool.map(cpu_worker, jobs)
+    tph = sum(rate * 3600.0 / n for (rate, _), (_, n, _) in zip(out, jobs))
+    us = 1e6 / (sum(r for r, _ in out) / len(out))
+    return {
+        "value": round(tph, 3),
+        "unit": "tests/hour",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"c2p restatement (oracle/kite_oracle.c), one test per core, {secs:.0f} s of squarings per "
+                  f"core at L={out[0][1]} ({us:.0f} us/squaring/core), extrapolated as rate*3600/n",
+    }
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this command, if any."""
+    p = os.path.join(ROOT, "profiles", f"{kernel}_pmc_single_large_batch.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=3000)
+    ap.add_argument("--cpu-secs", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (independent tests shard "
+              f"across GPUs with no exchange; run one process per GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    from ahnum_amd import kite, workloads
+
+    tests = workloads.single_large(a.batch)
+
+    # ---- plain path: one test at a time through kite.h ----
+    k0, n0 = tests[0]
+    with kite.Ctx(k0, n0) as c:
+        L0 = c.L
+        x = c.num(workloads.random_residue_limbs(c.N, 2))
+        x.square(a.warmup)
+        c.maxerr()
+        c.reset_maxerr()
+        t0 = time.perf_counter()
+        x.square(a.plain_steps)
+        plain_err = c.maxerr()  # waits for the squarings
+        plain_s = time.perf_counter() - t0
+        x.free()
+    plain_us = plain_s * 1e6 / a.plain_steps
+    plain_tph = 3600.0 / (n0 * plain_us * 1e-6)
+
+    # ---- batched engine ----
+    b = kite.Batch()
+    Ls = []
+    for i, (k, n) in enumerate(tests):
+        s = b.add(k, n)
+        b.set(s, 0, workloads.random_residue_limbs(k * (1 << n) - 1, 100 + i))
+        Ls.append(b.fft_len(s))
+    b.square(a.warmup)
+    b.maxerr(reset=True)
+    b.timing(True)
+    b.sync()
+    t0 = time.perf_counter()
+    b.square(a.steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    ms, launches = b.pass_times()
+    errs = b.maxerr()
+    b.close()
+
+    rate = a.steps / wall  # squarings/s per test
+    tph = sum(rate * 3600.0 / n for _, n in tests)
+    names = ["k_rows", "k_cols_inv", "k_cols_fwd"]
+    per_test_bytes = [16, 8, 8]
+    dom = max(range(3), key=lambda p: ms[p])
+    avg_s = ms[dom] / 1e3 / launches[dom]
+    alg_bytes = sum(per_test_bytes[dom] * L for L in Ls)
+    achieved = alg_bytes / avg_s / 1e9
+    traffic = load_traffic(names[dom])
+
+    out = {
+        "metric": "tests/hour at n~660K (Fermat base 3, FFT 49152, Gerbicz-checkable residues), batched",
+        "value": round(tph, 1),
+        "unit": "tests/hour",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random residues mod k*2^n-1)",
+        "config": {
+            "workload": "single_large batch: B Fermat base-3 tests, k odd in [1001,9999], n in [650000,670000]",
+            "batch": a.batch,
+            "fft_lengths": sorted(set(Ls)),
+            "seed": 2,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": names[dom],
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "pass_ms": {names[p]: round(ms[p], 3) for p in range(3)},
+        },
+        "plain": {
+            "us_per_squaring": round(plain_us, 3),
+            "tests_per_hour": round(plain_tph, 2),
+            "fft_len": L0,
+            "maxerr": round(plain_err, 4),
+            "number": f"{k0}*2^{n0}-1",
+        },
+        "batched_us_per_test_squaring": round(wall * 1e6 / a.steps / a.batch, 4),
+        "batched_vs_plain": round(tph / plain_tph, 2),
+        "maxerr": round(max(errs.values()), 4),
+        "all_gpu_ms_per_step": round(sum(ms) / a.steps, 4),
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(tests[:16], a.cpu_secs)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

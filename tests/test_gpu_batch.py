@@ -1,0 +1,101 @@
+"""The batched engine: many tests of different k, n and FFT length advance through shared launches.
+Each result must equal the exact value and the test's result alone."""
+import os
+import random
+import sys
+
+import pytest
+
+from ahnum_amd import kite, workloads
+from test_gpu_engine import mod_riesel, sqr_n
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_mixed_lengths_one_launch():
+    """Slots of different lengths (and shapes) squared by the same launches, with different
+    squaring counts per slot; every value exact."""
+    cases = [(4523, 67231), (9357, 20000), (1, 44497), (3, 26459), (8045, 659321), (9999, 30011), (7, 4001)]
+    b = kite.Batch()
+    r = random.Random(3)
+    vals = {}
+    try:
+        for k, n in cases:
+            s = b.add(k, n)
+            a = r.randrange(k * (1 << n) - 1)
+            b.set(s, 0, a)
+            vals[s] = (k, n, a)
+        say("lengths", sorted({b.fft_len(s) for s in vals}))
+        cap = b.lib.kite_batch_capacity(b.p)
+        import ctypes
+        which = (ctypes.c_int32 * cap)(*[0] * cap)
+        counts = [5 + 3 * s for s in range(cap)]
+        cnt = (ctypes.c_uint32 * cap)(*counts)
+        assert b.lib.kite_batch_square(b.p, which, cnt) == 0
+        for s, (k, n, a) in vals.items():
+            assert b.get(s, 0) == sqr_n(a, counts[s], k, n), (k, n)
+        errs = b.maxerr()
+        assert max(errs.values()) < 0.4
+    finally:
+        b.close()
+
+
+def test_batch_prp_continuous_admission_vs_gmp(golden):
+    """The 16 seeded n~65K tests through 5 slots (tests admitted as others finish); RES64s equal
+    GMP's, i.e. the same as each test alone."""
+    rows = golden("res64_plain_small.csv")
+    tests = [(int(k), int(n)) for k, n, _, _ in rows]
+    done = []
+    rc, out = kite.batch_prp(tests, slots=5, block=1000, on_done=lambda i, st, r: done.append(i))
+    assert rc == 0
+    assert sorted(done) == list(range(16))
+    for (k, n, base, want), (st, r) in zip(rows, out):
+        assert st == 0
+        assert "%016X" % r.res64 == want, (k, n)
+        assert r.maxerr < 0.4
+
+
+def test_batch_prp_fault_caught():
+    """A +1 digit corruption in one test of the batch is caught by its Gerbicz check; every
+    result stays right (CMakeLists.txt:95's case, plus neighbours)."""
+    tests = [(1203, 90001), (9357, 20000), (8191, 25000)]
+    rc, out = kite.batch_prp(tests, block=1000, fault_test=0, fault_at=50000)
+    assert rc == 0
+    assert "%016X" % out[0][1].res64 == "4BE9F1953E350085"
+    assert out[0][1].gerbicz_errors >= 1
+    assert "%016X" % out[1][1].res64 == "79291CD2F47CE48C"
+    assert "%016X" % out[2][1].res64 == "D45538E6F07266C6"
+    assert out[1][1].gerbicz_errors == 0
+
+
+def test_batch_prp_primecrunch_rows(golden):
+    """The reference's 14 primecrunch residues near 9357*2^369717-1 (tests/known_residues.csv:14-27),
+    all in one batch."""
+    rows = [r for r in golden("known_residues.csv") if r[3] == "primecrunch"]
+    assert len(rows) == 14
+    tests = [(int(k), int(n)) for k, n, _, _ in rows]
+    say("batch of", len(tests), "at n~370K")
+    rc, out = kite.batch_prp(tests, block=1000)
+    assert rc == 0
+    for (k, n, want, _), (st, r) in zip(rows, out):
+        assert st == 0 and "%016X" % r.res64 == want, (k, n)
+        assert r.maxerr < 0.4
+
+
+def test_batch_subset_vs_gmp(golden):
+    """Seeded tests of the 256-test mixed batch (k odd in [3, 9999], n in [300K, 700K]) whose GMP
+    RES64s are committed (tests/golden/res64_batch_subset.csv)."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "res64_batch_subset.csv")
+    if not os.path.exists(path):
+        pytest.skip("GMP fixture for the mixed batch not generated yet")
+    rows = golden("res64_batch_subset.csv")
+    tests = [(int(k), int(n)) for k, n, _, _ in rows]
+    assert tests == workloads.batch_mixed()[: len(tests)]
+    rc, out = kite.batch_prp(tests, block=1000)
+    assert rc == 0
+    for (k, n, base, want), (st, r) in zip(rows, out):
+        assert st == 0 and "%016X" % r.res64 == want, (k, n)
