@@ -32,13 +32,14 @@
 #include <vector>
 
 #include "engine_dev.h"
+#include "kernels.h"
 #include "kite_batch.h"
 #include "kite_internal.h"
 
-typedef unsigned __int128 u128;
-typedef __int128 i128;
 
 namespace kite {
+
+typedef unsigned __int128 u128;
 
 static thread_local std::string g_err;
 const char *engine_last_error() { return g_err.c_str(); }
@@ -54,462 +55,8 @@ const char *engine_last_error() { return g_err.c_str(); }
     } while (0)
 
 // ================================================================================================
-// Device code
+// Utility kernels (the pass kernels live in fft_dev.h, kernels_generic.hip, kernels_fast.hip)
 // ================================================================================================
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
-    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-// multiply by -i (forward) or +i (inverse)
-template <bool INV> __device__ __forceinline__ double2 mul_mi(double2 a) {
-    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
-}
-
-// ---- small DFTs, y_k = sum_j x_j e^{-+2 pi i jk/p} ----------------------------------------------
-
-template <int P, bool INV> struct Dft;
-
-template <bool INV> struct Dft<2, INV> {
-    static __device__ __forceinline__ void run(double2 *v) {
-        double2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    }
-};
-
-template <bool INV> struct Dft<4, INV> {
-    static __device__ __forceinline__ void run(double2 *v) {
-        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        double2 s13 = cadd(v[1], v[3]), d13 = mul_mi<INV>(csub(v[1], v[3]));
-        v[0] = cadd(s02, s13);
-        v[2] = csub(s02, s13);
-        v[1] = cadd(d02, d13);
-        v[3] = csub(d02, d13);
-    }
-};
-
-template <bool INV> struct Dft<8, INV> {
-    static __device__ __forceinline__ void run(double2 *v) {
-        const double h = 0.70710678118654752440;
-        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-        Dft<4, INV>::run(e);
-        Dft<4, INV>::run(o);
-        // o_k *= w8^k, w8 = e^{-+ i pi/4}
-        double2 o1 = INV ? make_double2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
-                         : make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
-        double2 o2 = mul_mi<INV>(o[2]);
-        double2 o3 = INV ? make_double2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
-                         : make_double2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
-        v[0] = cadd(e[0], o[0]);
-        v[4] = csub(e[0], o[0]);
-        v[1] = cadd(e[1], o1);
-        v[5] = csub(e[1], o1);
-        v[2] = cadd(e[2], o2);
-        v[6] = csub(e[2], o2);
-        v[3] = cadd(e[3], o3);
-        v[7] = csub(e[3], o3);
-    }
-};
-
-// odd p: pairs (x_j, x_{p-j}) with s = sum, d = difference; y_k = A -+ iB, y_{p-k} = A +- iB
-template <int P> struct OddRoots;
-template <> struct OddRoots<3> {
-    static __device__ __forceinline__ double c(int q) { return -0.5; }
-    static __device__ __forceinline__ double s(int q) { return q == 1 ? 0.86602540378443864676 : -0.86602540378443864676; }
-};
-template <> struct OddRoots<5> {
-    static __device__ __forceinline__ double c(int q) {
-        return (q == 1 || q == 4) ? 0.30901699437494742410 : -0.80901699437494742410;
-    }
-    static __device__ __forceinline__ double s(int q) {
-        return q == 1 ? 0.95105651629515357212 : q == 2 ? 0.58778525229247312917
-             : q == 3 ? -0.58778525229247312917 : -0.95105651629515357212;
-    }
-};
-template <> struct OddRoots<7> {
-    static __device__ __forceinline__ double c(int q) {
-        return (q == 1 || q == 6) ? 0.62348980185873353053
-             : (q == 2 || q == 5) ? -0.22252093395631440429 : -0.90096886790241912624;
-    }
-    static __device__ __forceinline__ double s(int q) {
-        return q == 1 ? 0.78183148246802980871 : q == 2 ? 0.97492791218182360702
-             : q == 3 ? 0.43388373911755812048 : q == 4 ? -0.43388373911755812048
-             : q == 5 ? -0.97492791218182360702 : -0.78183148246802980871;
-    }
-};
-
-template <int P, bool INV> struct DftOdd {
-    static __device__ __forceinline__ void run(double2 *v) {
-        constexpr int M = (P - 1) / 2;
-        double2 sp[M], dm[M];
-        double2 y0 = v[0];
-#pragma unroll
-        for (int j = 1; j <= M; j++) {
-            sp[j - 1] = cadd(v[j], v[P - j]);
-            dm[j - 1] = csub(v[j], v[P - j]);
-            y0 = cadd(y0, sp[j - 1]);
-        }
-        double2 out[P];
-        out[0] = y0;
-#pragma unroll
-        for (int k = 1; k <= M; k++) {
-            double2 A = v[0], B = make_double2(0, 0);
-#pragma unroll
-            for (int j = 1; j <= M; j++) {
-                const int q = (j * k) % P;
-                const double cq = OddRoots<P>::c(q), sq = OddRoots<P>::s(q);
-                A.x += sp[j - 1].x * cq;
-                A.y += sp[j - 1].y * cq;
-                B.x += dm[j - 1].x * sq;
-                B.y += dm[j - 1].y * sq;
-            }
-            // forward: y_k = A - iB, y_{P-k} = A + iB
-            double2 miB = make_double2(B.y, -B.x);
-            out[k] = INV ? csub(A, miB) : cadd(A, miB);
-            out[P - k] = INV ? cadd(A, miB) : csub(A, miB);
-        }
-#pragma unroll
-        for (int k = 0; k < P; k++) v[k] = out[k];
-    }
-};
-template <bool INV> struct Dft<3, INV> : DftOdd<3, INV> {};
-template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
-template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
-
-// ---- Stockham stage in LDS ------------------------------------------------------------------
-// `howmany` sequences of length n; element (seq, idx) at buf[idx*is + seq*ss]. Stage with radix P
-// after Ns points have been combined: butterfly j reads idx j + r*n/P, twiddles by
-// w_{Ns P}^{(j mod Ns) r} = tw[(j mod Ns) r n/(Ns P)], writes idx (j/Ns) Ns P + (j mod Ns) + r Ns.
-template <int P, bool INV>
-__device__ __forceinline__ void fft_stage(const double2 *__restrict__ src, double2 *__restrict__ dst,
-                                          int n, int Ns, int howmany, int is, int ss, bool seqfast,
-                                          const double2 *__restrict__ tw) {
-    const int nb = n / P, total = nb * howmany, ts = n / (Ns * P);
-    for (int w = threadIdx.x; w < total; w += blockDim.x) {
-        int seq, j;
-        if (seqfast) {
-            seq = w % howmany;
-            j = w / howmany;
-        } else {
-            j = w % nb;
-            seq = w / nb;
-        }
-        const int jm = j % Ns;
-        double2 v[P];
-#pragma unroll
-        for (int r = 0; r < P; r++) v[r] = src[(j + r * nb) * is + seq * ss];
-        if (jm) {
-#pragma unroll
-            for (int r = 1; r < P; r++) {
-                double2 t = tw[jm * r * ts];
-                v[r] = INV ? cmulc(v[r], t) : cmul(v[r], t);
-            }
-        }
-        Dft<P, INV>::run(v);
-        const int base = (j / Ns) * Ns * P + jm;
-#pragma unroll
-        for (int r = 0; r < P; r++) dst[(base + r * Ns) * is + seq * ss] = v[r];
-    }
-}
-
-template <bool INV>
-__device__ double2 *lds_fft(double2 *a, double2 *b, const Plan &pl, int howmany, int is, int ss,
-                            bool seqfast, const double2 *tw) {
-    int Ns = 1;
-    for (int s = 0; s < pl.nst; s++) {
-        const int p = pl.rad[s];
-        switch (p) {
-        case 2: fft_stage<2, INV>(a, b, pl.n, Ns, howmany, is, ss, seqfast, tw); break;
-        case 3: fft_stage<3, INV>(a, b, pl.n, Ns, howmany, is, ss, seqfast, tw); break;
-        case 4: fft_stage<4, INV>(a, b, pl.n, Ns, howmany, is, ss, seqfast, tw); break;
-        case 5: fft_stage<5, INV>(a, b, pl.n, Ns, howmany, is, ss, seqfast, tw); break;
-        case 7: fft_stage<7, INV>(a, b, pl.n, Ns, howmany, is, ss, seqfast, tw); break;
-        default: fft_stage<8, INV>(a, b, pl.n, Ns, howmany, is, ss, seqfast, tw); break;
-        }
-        __syncthreads();
-        double2 *t = a;
-        a = b;
-        b = t;
-        Ns *= p;
-    }
-    return a;
-}
-
-// w_H^t for t < H from the two-level table
-__device__ __forceinline__ double2 tw_h(const ShapeDev &sh, int t) {
-    return cmul(sh.thi[t >> sh.sshift], sh.tlo[t & ((1 << sh.sshift) - 1)]);
-}
-
-// ---- real-split pointwise square / multiply ----------------------------------------------------
-// a = U_k, b = U_{H-k}, t = w_H^k; e = a + conj(b), o = -i (a - conj(b)).
-// square: X = e^2 + t o^2, Y = 2 e o; multiply by the operand's (ya, yb): X = e ey + t o oy,
-// Y = e oy + o ey. New a = X + iY, new b = conj(X) + i conj(Y) (4x the packed product's DFT).
-__device__ __forceinline__ void split_sq(double2 &a, double2 &b, double2 t) {
-    double2 e = make_double2(a.x + b.x, a.y - b.y);
-    double2 o = make_double2(a.y + b.y, b.x - a.x);
-    double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2 * e.x * e.y);
-    double2 o2 = make_double2(o.x * o.x - o.y * o.y, 2 * o.x * o.y);
-    double2 X = cadd(e2, cmul(t, o2));
-    double2 Y = make_double2(2 * (e.x * o.x - e.y * o.y), 2 * (e.x * o.y + e.y * o.x));
-    a = make_double2(X.x - Y.y, X.y + Y.x);
-    b = make_double2(X.x + Y.y, Y.x - X.y);
-}
-
-__device__ __forceinline__ void split_mul(double2 &a, double2 &b, double2 t, double2 ya, double2 yb) {
-    double2 e = make_double2(a.x + b.x, a.y - b.y);
-    double2 o = make_double2(a.y + b.y, b.x - a.x);
-    double2 ey = make_double2(ya.x + yb.x, ya.y - yb.y);
-    double2 oy = make_double2(ya.y + yb.y, yb.x - ya.x);
-    double2 X = cadd(cmul(e, ey), cmul(t, cmul(o, oy)));
-    double2 Y = cadd(cmul(e, oy), cmul(o, ey));
-    a = make_double2(X.x - Y.y, X.y + Y.x);
-    b = make_double2(X.x + Y.y, Y.x - X.y);
-}
-
-__device__ __forceinline__ const SlotDev &pick_slot(const SlotDev *slots, const SlotDev &one, int i) {
-    return slots ? slots[i] : one;
-}
-
-// rows handled by row job u: (0, R/2) for u = 0 (both self-paired), else (u, R-u)
-__device__ __forceinline__ void row_pair(int R, int u, int &ra, int &rb) {
-    if (u == 0) {
-        ra = 0;
-        rb = (R % 2 == 0 && R > 1) ? R / 2 : -1;
-    } else {
-        ra = u;
-        rb = R - u;
-    }
-}
-
-// ---- kernels -----------------------------------------------------------------------------------
-
-extern "C" __global__ void __launch_bounds__(kNT) k_rows(const SlotDev *slots, SlotDev one,
-                                                      const int2 *jobs, int mode, int step) {
-    extern __shared__ double2 lds[];
-    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
-    if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
-    const ShapeDev &sh = *s.shp;
-    const int R = sh.R, C = sh.C;
-    int ra, rb;
-    row_pair(R, jb.y, ra, rb);
-    const int nrows = rb >= 0 ? 2 : 1;
-    double2 *A = lds, *B = lds + 2 * C;
-    double2 *x = mode == kModeRowFft ? const_cast<double2 *>(s.y_src) : s.x;
-    for (int i = threadIdx.x; i < nrows * C; i += blockDim.x) {
-        const int sr = i >= C, c = i - sr * C;
-        A[i] = x[(size_t)(sr ? rb : ra) * C + c];
-    }
-    __syncthreads();
-    double2 *F = lds_fft<false>(A, B, sh.pc, nrows, 1, C, false, sh.twC);
-    if (mode == kModeRowFft) {  // operand preparation: store the row spectra
-        for (int i = threadIdx.x; i < nrows * C; i += blockDim.x) {
-            const int sr = i >= C, c = i - sr * C;
-            s.yfreq[(size_t)(sr ? rb : ra) * C + c] = F[i];
-        }
-        return;
-    }
-    const bool mul = mode == kModeMul;
-    const double2 *Y = s.yfreq;
-    if (jb.y == 0) {
-        // self-paired rows: row 0 pairs k_c with (C-k_c) mod C, row R/2 pairs k_c with C-1-k_c
-        for (int i = threadIdx.x; i < nrows * C; i += blockDim.x) {
-            const int sr = i >= C, kc = i - sr * C;
-            const int pc = sr ? C - 1 - kc : (C - kc) % C;
-            if (pc < kc) continue;
-            const int kr = sr ? rb : 0;
-            double2 a = F[sr * C + kc], b = F[sr * C + pc];
-            const double2 t = tw_h(sh, kr + R * kc);
-            if (mul) split_mul(a, b, t, Y[(size_t)kr * C + kc], Y[(size_t)kr * C + pc]);
-            else split_sq(a, b, t);
-            F[sr * C + kc] = a;
-            if (pc != kc) F[sr * C + pc] = b;
-        }
-    } else {
-        for (int kc = threadIdx.x; kc < C; kc += blockDim.x) {
-            const int pc = C - 1 - kc;
-            double2 a = F[kc], b = F[C + pc];
-            const double2 t = tw_h(sh, ra + R * kc);
-            if (mul) split_mul(a, b, t, Y[(size_t)ra * C + kc], Y[(size_t)rb * C + pc]);
-            else split_sq(a, b, t);
-            F[kc] = a;
-            F[C + pc] = b;
-        }
-    }
-    __syncthreads();
-    double2 *Gb = lds_fft<true>(F, F == A ? B : A, sh.pc, nrows, 1, C, false, sh.twC);
-    for (int i = threadIdx.x; i < nrows * C; i += blockDim.x) {
-        const int sr = i >= C, c = i - sr * C;
-        s.x[(size_t)(sr ? rb : ra) * C + c] = Gb[i];
-    }
-}
-
-// floor(h / k) for 0 < k < 2^32
-__device__ __forceinline__ i128 floordiv_small(i128 h, uint32_t k) {
-    const bool neg = h < 0;
-    u128 a = neg ? (u128)(-h) : (u128)h;
-    uint64_t hi = (uint64_t)(a >> 64), lo = (uint64_t)a;
-    uint64_t qh = hi / k, r = hi % k;
-    uint64_t t = (r << 32) | (lo >> 32);
-    uint64_t q1 = t / k;
-    r = t % k;
-    t = (r << 32) | (lo & 0xffffffffull);
-    uint64_t q0 = t / k;
-    r = t % k;
-    u128 q = ((u128)qh << 64) | ((u128)q1 << 32) | q0;
-    if (!neg) return (i128)q;
-    return -(i128)q - (r ? 1 : 0);
-}
-
-// Fold the top region (digits top_t..L-1 of the last chunk plus the carry at 2^{s_L}):
-// H = q*k + r  ->  q into digit 0, r*2^m rebalanced into the top digits (src/ctx.c:345-375).
-__device__ int64_t fold_top_dev(const SlotDev &s, int32_t *d, int64_t carry) {
-    const int t0 = s.top_i0, nt = s.top_n, m = s.top_m;
-    i128 Hh = (i128)carry << s.top_shift;
-    int pos = 0;
-    for (int i = 0; i < nt; i++) {
-        Hh += (i128)d[t0 + i] << pos;
-        pos += s.top_bits[i];
-    }
-    const i128 K = (i128)s.k;
-    const i128 lo = Hh & (((i128)1 << m) - 1), hi = Hh >> m;
-    i128 q = floordiv_small(hi, (uint32_t)s.k);
-    i128 top = lo + ((hi - q * K) << m);
-    if (2 * top >= (K << m)) {
-        top -= K << m;
-        q += 1;
-    }
-    for (int i = 0; i + 1 < nt; i++) {
-        const int b = s.top_bits[i];
-        const i128 qq = (top + ((i128)1 << (b - 1))) >> b;
-        d[t0 + i] = (int32_t)(top - (qq << b));
-        top = qq;
-    }
-    if (nt > 0) d[t0 + nt - 1] = (int32_t)top;
-    return (int64_t)q;
-}
-
-__device__ __forceinline__ double pow2i(int e) { return e == 0 ? 1.0 : e == 1 ? 2.0 : 4.0; }
-__device__ __forceinline__ double pow2ni(int e) { return e == 0 ? 1.0 : e == 1 ? 0.5 : 0.25; }
-
-extern "C" __global__ void __launch_bounds__(kNT) k_cols_inv(const SlotDev *slots, SlotDev one,
-                                                      const int2 *jobs, int mode, int step) {
-    extern __shared__ double2 lds[];
-    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
-    if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
-    const ShapeDev &sh = *s.shp;
-    const int R = sh.R, C = sh.C, G = sh.G, NG = sh.NG, g = jb.y, c0 = g * G;
-    double2 *A = lds, *B = lds + R * G;
-    for (int i = threadIdx.x; i < R * G; i += blockDim.x) {
-        const int kr = i / G, l = i - kr * G;
-        A[i] = cmulc(s.x[(size_t)kr * C + c0 + l], tw_h(sh, (c0 + l) * kr));
-    }
-    __syncthreads();
-    double2 *F = lds_fft<true>(A, B, sh.pr, G, G, 1, true, sh.twR);
-    int32_t *D = reinterpret_cast<int32_t *>(F == A ? B : A);
-    const bool sq = mode != kModeDigits;
-    const double sc0 = sq ? s.kscale : s.rscale;
-    const uint8_t *code = s.dcode + (size_t)g * R * G;
-    double lmax = 0;
-    for (int r = threadIdx.x; r < R; r += blockDim.x) {
-        const double tap = s.ta[R + r];
-        int64_t cy = 0;
-        int32_t *dr = D + r * 2 * G;
-        for (int l = 0; l < G; l++) {
-            const double2 v = F[r * G + l];
-            const int cd = code[r * G + l];
-            const int cb = 2 * (c0 + l);
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int ce = h ? (cd >> 3) & 3 : cd & 3;
-                const int big = h ? (cd >> 5) & 1 : (cd >> 2) & 1;
-                const double x = (h ? v.y : v.x) * (sc0 * tap * s.tb[2 * C + cb + h] * pow2ni(ce));
-                const double rx = rint(x);
-                lmax = fmax(lmax, fabs(x - rx));
-                if (sq) {
-                    const int b = s.b0 + big;
-                    const int64_t t = (int64_t)rx + cy;
-                    const int64_t q = (t + ((int64_t)1 << (b - 1))) >> b;
-                    dr[2 * l + h] = (int32_t)(t - (q << b));
-                    cy = q;
-                } else {
-                    dr[2 * l + h] = (int32_t)rx;
-                }
-            }
-        }
-        if (sq) {
-            if (g == NG - 1 && r == R - 1) cy = fold_top_dev(s, dr, cy);
-            s.cout[g * R + r] = cy;
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
-    if ((threadIdx.x & 63) == 0 && lmax > 0)
-        atomicMax(s.maxerr, (unsigned long long)__double_as_longlong(lmax));
-    __syncthreads();
-    int32_t *out = s.dig + (size_t)g * R * 2 * G;
-    for (int i = threadIdx.x; i < R * 2 * G; i += blockDim.x) out[i] = D[i];
-}
-
-extern "C" __global__ void __launch_bounds__(kNT) k_cols_fwd(const SlotDev *slots, SlotDev one,
-                                                      const int2 *jobs, int mode, int step) {
-    extern __shared__ double2 lds[];
-    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
-    if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
-    const ShapeDev &sh = *s.shp;
-    const int R = sh.R, C = sh.C, G = sh.G, NG = sh.NG, g = jb.y, c0 = g * G;
-    double2 *A = lds, *B = lds + R * G;
-    int32_t *D = reinterpret_cast<int32_t *>(B);
-    const int32_t *in = s.dig + (size_t)g * R * 2 * G;
-    for (int i = threadIdx.x; i < R * 2 * G; i += blockDim.x) D[i] = in[i];
-    __syncthreads();
-    const uint8_t *code = s.dcode + (size_t)g * R * G;
-    if (mode != kModeDigits) {
-        for (int r = threadIdx.x; r < R; r += blockDim.x) {
-            int pidx;  // chunk before (g, r) in digit order, cyclically
-            if (g > 0) pidx = (g - 1) * R + r;
-            else if (r > 0) pidx = (NG - 1) * R + r - 1;
-            else pidx = (NG - 1) * R + R - 1;
-            int64_t cy = s.cout[pidx];
-            int32_t *dr = D + r * 2 * G;
-            for (int i = 0; i < 2 * G && cy; i++) {
-                if (i == 2 * G - 1) {
-                    dr[i] = (int32_t)(dr[i] + cy);
-                    break;
-                }
-                const int cd = code[r * G + (i >> 1)];
-                const int big = (i & 1) ? (cd >> 5) & 1 : (cd >> 2) & 1;
-                const int b = s.b0 + big;
-                const int64_t t = (int64_t)dr[i] + cy;
-                const int64_t q = (t + ((int64_t)1 << (b - 1))) >> b;
-                dr[i] = (int32_t)(t - (q << b));
-                cy = q;
-            }
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < R * G; i += blockDim.x) {
-        const int r = i / G, l = i - r * G;
-        const int cd = code[i];
-        const double tam = s.ta[r];
-        const int cb = 2 * (c0 + l);
-        const double w0 = tam * s.tb[cb] * pow2i(cd & 3);
-        const double w1 = tam * s.tb[cb + 1] * pow2i((cd >> 3) & 3);
-        A[i] = make_double2((double)D[r * 2 * G + 2 * l] * w0, (double)D[r * 2 * G + 2 * l + 1] * w1);
-    }
-    __syncthreads();
-    double2 *F = lds_fft<false>(A, B, sh.pr, G, G, 1, true, sh.twR);
-    for (int i = threadIdx.x; i < R * G; i += blockDim.x) {
-        const int kr = i / G, l = i - kr * G;
-        s.x[(size_t)kr * C + c0 + l] = cmul(F[i], tw_h(sh, (c0 + l) * kr));
-    }
-}
 
 extern "C" __global__ void k_addsub(double2 *x, const double2 *y, int n, double sign) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -572,11 +119,12 @@ static bool smooth7(uint32_t n) {
     return n == 1;
 }
 
-// Choose H = R x C and the column group width G for length L (top_digits = L - top_t, 0 if
-// unknown). Rows pass: 2 rows of C in LDS (ping-pong: 64 C bytes); column pass: G columns of R
-// (32 G R bytes). Preference: C about 4R (balanced passes), larger G for coalesced strips.
+// Choose H = R x C for length L (top_digits = L - top_t, 0 if unknown). The column group width
+// is g_for_R(R); the top region must fit the last 16-digit carry sub-chunk. Preference: C about
+// 4R (balanced passes), and sizes with compile-time kernels (kernels_fast.hip).
 struct ShapeChoice {
     uint32_t R = 0, C = 0, G = 0;
+    bool fast = false;
 };
 
 static bool choose_shape(uint32_t L, uint32_t top_digits, uint32_t variant, ShapeChoice *out) {
@@ -588,18 +136,19 @@ static bool choose_shape(uint32_t L, uint32_t top_digits, uint32_t variant, Shap
         if (H % C || C % 4 || !smooth7(C)) continue;
         const uint32_t R = H / C;
         if (R > kMaxR || !smooth7(R)) continue;
-        uint32_t G = 0;
-        for (uint32_t g = 32; g >= 4; g /= 2)
-            if (C % g == 0 && (uint64_t)g * R * 32 <= kMaxColLds) { G = g; break; }
-        if (!G) continue;
-        if (top_digits && 2 * G < top_digits) continue;
-        double cost = fabs(log2((double)C) - log2((double)R) - 2.0);
-        if (G < 8) cost += 2;
-        cands.push_back({R, C, G});
+        uint32_t G = (uint32_t)g_for_R((int)R);
+        const bool gfull = C % G == 0;  // the compile-time kernels use exactly g_for_R(R)
+        while (C % G) G /= 2;           // the generic kernels take any power of two dividing C
+        const uint32_t sub = G < (uint32_t)kSub ? G : (uint32_t)kSub;
+        if (top_digits && 2 * sub < top_digits) continue;
+        if ((2ull * R * G + R) * 16 + (uint64_t)R * (G / sub) * 8 > (uint64_t)kMaxLds) continue;
+        const bool fast = gfull && fast_rows((int)C) && fast_cols_inv((int)R);
+        double cost = fabs(log2((double)C) - log2((double)R) - 2.0) + (fast ? 0 : 4);
+        cands.push_back({R, C, G, fast});
         costs.push_back(cost);
     }
     if (cands.empty()) return false;
-    // rank by cost; `variant` picks the (variant)-th best (autotuner), 0 = best
+    // rank by cost; `variant` picks the (variant)-th best (the autotuner's shape order), 0 = best
     std::vector<size_t> idx(cands.size());
     for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
     std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return costs[a] < costs[b]; });
@@ -611,6 +160,11 @@ static bool choose_shape(uint32_t L, uint32_t top_digits, uint32_t variant, Shap
 bool engine_shape_ok(uint32_t L, uint32_t top_digits) {
     ShapeChoice sc;
     return choose_shape(L, top_digits, 0, &sc);
+}
+
+bool engine_shape_fast(uint32_t L) {
+    ShapeChoice sc;
+    return choose_shape(L, 0, 0, &sc) && sc.fast;
 }
 
 // Shapes (twiddle tables) are shared by every context of the same (R, C, G) on a device.
@@ -655,9 +209,10 @@ static ShapeEntry *get_shape(const ShapeChoice &ch) {
     while ((1ull << (2 * sh)) < H) sh++;
     h.sshift = sh;
     const uint64_t S = 1ull << sh, nhi = (H + S - 1) / S;
-    double2 *twR, *twC, *tlo, *thi;
+    double2 *twR, *twC, *tlo, *thi, *twH;
     if (upload(roots_table(ch.R, ch.R, 1), &twR, e->bufs) || upload(roots_table(ch.C, ch.C, 1), &twC, e->bufs) ||
-        upload(roots_table(H, S, 1), &tlo, e->bufs) || upload(roots_table(H, nhi, S), &thi, e->bufs)) {
+        upload(roots_table(H, S, 1), &tlo, e->bufs) || upload(roots_table(H, nhi, S), &thi, e->bufs) ||
+        upload(roots_table(H, H, 1), &twH, e->bufs)) {
         delete e;
         return nullptr;
     }
@@ -665,6 +220,7 @@ static ShapeEntry *get_shape(const ShapeChoice &ch) {
     h.twC = twC;
     h.tlo = tlo;
     h.thi = thi;
+    h.twH = twH;
     std::vector<ShapeDev> one(1, h);
     if (upload(one, &e->dev, e->bufs)) {
         delete e;
@@ -687,6 +243,7 @@ struct Engine {
     std::vector<int32_t> hdig;
     int nrowjobs = 0, ncoljobs = 0;
     size_t lds_rows = 0, lds_cols = 0;
+    PassKernel krows = nullptr, kcinv = nullptr, kcfwd = nullptr;
     uint32_t H = 0;
 };
 
@@ -694,9 +251,11 @@ static std::once_flag g_attr_once;
 
 static void set_lds_limits() {
     std::call_once(g_attr_once, [] {
-        hipFuncSetAttribute((const void *)k_rows, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-        hipFuncSetAttribute((const void *)k_cols_inv, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-        hipFuncSetAttribute((const void *)k_cols_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+        PassKernel ks[256];
+        int n = 0;
+        all_kernels(ks, &n, 256);
+        for (int i = 0; i < n; i++)
+            if (ks[i]) hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
     });
 }
 
@@ -742,13 +301,14 @@ int build_slot_tables(kite_ctx *ctx, const ShapeDev &sh, std::vector<double> &ta
 }
 
 int fill_slot_fold(kite_ctx *ctx, const ShapeDev &sh, SlotDev *s) {
-    const uint32_t L = ctx->L, chunk0 = L - 2 * sh.G;
+    const uint32_t sub = sh.G < kSub ? (uint32_t)sh.G : (uint32_t)kSub;
+    const uint32_t L = ctx->L, chunk0 = L - 2 * sub;  // the fold works on the last sub-chunk
     s->k = ctx->k;
     s->b0 = (int)ctx->b0;
     s->top_n = (int)(L - ctx->top_t);
-    if (s->top_n > 2 * sh.G || s->top_n > kMaxTop) return -1;
+    if (s->top_n > (int)(2 * sub) || s->top_n > kMaxTop) return -1;
     s->top_i0 = (int)(ctx->top_t - chunk0);
-    if (ctx->top_t == L) s->top_i0 = 2 * sh.G;
+    if (ctx->top_t == L) s->top_i0 = (int)(2 * sub);
     s->top_m = (int)ctx->top_m;
     s->top_shift = (int)(ctx->sL - ctx->top_st);
     for (int i = 0; i < s->top_n; i++) s->top_bits[i] = ctx->bits[ctx->top_t + i];
@@ -807,10 +367,27 @@ int engine_init(kite_ctx *ctx) {
         g_err = "top region does not fit the last chunk";
         return -1;
     }
-    E->nrowjobs = sh.R % 2 == 0 ? sh.R / 2 : (sh.R + 1) / 2;
-    E->ncoljobs = sh.NG;
-    E->lds_rows = (size_t)64 * sh.C;
-    E->lds_cols = (size_t)32 * sh.G * sh.R;
+    {
+        const int npairs = sh.R % 2 == 0 ? sh.R / 2 : (sh.R + 1) / 2;
+        const bool gct = sh.G == g_for_R(sh.R);  // compile-time column kernels assume this width
+        PassKernel fr = fast_rows(sh.C), fi = gct ? fast_cols_inv(sh.R) : nullptr,
+                   ff = gct ? fast_cols_fwd(sh.R) : nullptr;
+        const int ppw = fr ? ppw_for_C(sh.C) : 1;
+        E->krows = fr ? fr : generic_rows();
+        E->kcinv = fi ? fi : generic_cols_inv();
+        E->kcfwd = ff ? ff : generic_cols_fwd();
+        E->nrowjobs = (npairs + ppw - 1) / ppw;
+        E->ncoljobs = sh.NG;
+        const int sub = sh.G < kSub ? sh.G : kSub;
+        if (fr)  // single buffer of 2*ppw rows, then the row twiddles
+            E->lds_rows = (size_t)16 * (2 * ppw * sh.C + (sh.C <= (int)kTwLdsMaxC ? sh.C : 0));
+        else
+            E->lds_rows = (size_t)16 * (2 * 2 * sh.C);
+        if (fi)
+            E->lds_cols = fast_cols_lds(sh.R);
+        else
+            E->lds_cols = (size_t)16 * (2 * sh.R * sh.G + sh.R) + (size_t)8 * sh.R * (sh.G / sub);
+    }
     E->hdig.resize(ctx->L);
     return 0;
 }
@@ -844,21 +421,24 @@ void engine_copy(kite_ctx *ctx, void *dst, const void *src) {
     hipMemcpyAsync(dst, src, (size_t)ctx->eng->H * sizeof(double2), hipMemcpyDeviceToDevice, ctx->eng->stream);
 }
 
+static void launch_pass(PassKernel fn, int grid, size_t lds, hipStream_t st, const SlotDev *slots,
+                        const SlotDev &one, const int2 *jobs, int mode, int step) {
+    void *args[] = {(void *)&slots, (void *)&one, (void *)&jobs, (void *)&mode, (void *)&step};
+    hipLaunchKernel(fn, dim3(grid), dim3(kNT), args, lds, st);
+}
+
 static void launch_rows(Engine *E, double2 *x, const double2 *ysrc, int mode) {
     SlotDev s = E->slot;
     s.x = x;
     s.y_src = ysrc;
-    hipLaunchKernelGGL(k_rows, dim3(E->nrowjobs), dim3(kNT), E->lds_rows, E->stream, (const SlotDev *)nullptr, s,
-                       (const int2 *)nullptr, mode, 0);
+    launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, nullptr, s, nullptr, mode, 0);
 }
 
 static void launch_cols(Engine *E, double2 *x, int mode) {
     SlotDev s = E->slot;
     s.x = x;
-    hipLaunchKernelGGL(k_cols_inv, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, mode, 0);
-    hipLaunchKernelGGL(k_cols_fwd, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, mode, 0);
+    launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, mode, 0);
+    launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, mode, 0);
 }
 
 void engine_square(kite_ctx *ctx, void *x, uint64_t count) {
@@ -895,8 +475,7 @@ void engine_from_digits(kite_ctx *ctx, void *x, const int64_t *dig) {
     hipMemcpyAsync(E->dig, E->hdig.data(), (size_t)ctx->L * 4, hipMemcpyHostToDevice, E->stream);
     SlotDev s = E->slot;
     s.x = (double2 *)x;
-    hipLaunchKernelGGL(k_cols_fwd, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, (int)kModeDigits, 0);
+    launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, kModeDigits, 0);
     hipStreamSynchronize(E->stream);  // hdig is reused
 }
 
@@ -907,8 +486,7 @@ void engine_to_digits(kite_ctx *ctx, const void *x, int64_t *dig) {
     s.x = (double2 *)const_cast<void *>(x);
     // the conversion's own rounding error is not an operation's round-off: count it aside
     s.maxerr = E->maxerr + 1;
-    hipLaunchKernelGGL(k_cols_inv, dim3(E->ncoljobs), dim3(kNT), E->lds_cols, E->stream, (const SlotDev *)nullptr,
-                       s, (const int2 *)nullptr, (int)kModeDigits, 0);
+    launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, kModeDigits, 0);
     hipMemcpyAsync(E->hdig.data(), E->dig, (size_t)ctx->L * 4, hipMemcpyDeviceToHost, E->stream);
     hipStreamSynchronize(E->stream);
     const uint32_t R = sh.R, C = sh.C, G = sh.G, NG = sh.NG;
@@ -956,7 +534,12 @@ struct kite_batch {
     int2 *d_rows = nullptr, *d_cols = nullptr;
     int nrows = 0, ncols = 0;
     size_t jobs_cap = 0;
-    size_t lds_rows = 0, lds_cols = 0;
+    struct Group {
+        kite::PassKernel k1, k2;  // rows: k1; columns: k1 = inverse, k2 = forward
+        int off, count;
+        size_t lds;
+    };
+    std::vector<Group> grows, gcols;
     unsigned long long *d_maxerr = nullptr;
     bool dirty = true;
     bool timing = false;
@@ -986,17 +569,36 @@ static int batch_refresh(kite_batch *b) {
         }
         b->slot_cap = nc;
     }
+    // jobs grouped by kernel (one launch per group), groups contiguous in the job arrays
     std::vector<int2> rows, cols;
-    size_t lr = 0, lc = 0;
-    for (size_t s = 0; s < cap; s++) {
-        kite_ctx *ctx = b->slots[s].ctx;
-        if (!ctx) continue;
-        Engine *E = ctx->eng;
-        for (int u = 0; u < E->nrowjobs; u++) rows.push_back(make_int2((int)s, u));
-        for (int u = 0; u < E->ncoljobs; u++) cols.push_back(make_int2((int)s, u));
-        lr = std::max(lr, E->lds_rows);
-        lc = std::max(lc, E->lds_cols);
-    }
+    b->grows.clear();
+    b->gcols.clear();
+    std::vector<int> order;
+    for (size_t s = 0; s < cap; s++)
+        if (b->slots[s].ctx) order.push_back((int)s);
+    auto build = [&](bool isrows) {
+        std::vector<int2> &jobs = isrows ? rows : cols;
+        std::vector<kite_batch::Group> &grp = isrows ? b->grows : b->gcols;
+        std::vector<bool> done(cap, false);
+        for (int s0 : order) {
+            if (done[s0]) continue;
+            Engine *E0 = b->slots[s0].ctx->eng;
+            const PassKernel key = isrows ? E0->krows : E0->kcinv;
+            kite_batch::Group g{key, isrows ? nullptr : E0->kcfwd, (int)jobs.size(), 0, 0};
+            for (int s : order) {
+                Engine *E = b->slots[s].ctx->eng;
+                if (done[s] || (isrows ? E->krows : E->kcinv) != key) continue;
+                done[s] = true;
+                const int nj = isrows ? E->nrowjobs : E->ncoljobs;
+                for (int u = 0; u < nj; u++) jobs.push_back(make_int2(s, u));
+                g.lds = std::max(g.lds, isrows ? E->lds_rows : E->lds_cols);
+            }
+            g.count = (int)jobs.size() - g.off;
+            grp.push_back(g);
+        }
+    };
+    build(true);
+    build(false);
     const size_t need = std::max(rows.size(), cols.size());
     if (need > b->jobs_cap) {
         if (b->d_rows) hipFree(b->d_rows);
@@ -1010,8 +612,6 @@ static int batch_refresh(kite_batch *b) {
     if (!cols.empty()) HIP_OK(hipMemcpy(b->d_cols, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
     b->nrows = (int)rows.size();
     b->ncols = (int)cols.size();
-    b->lds_rows = lr;
-    b->lds_cols = lc;
     b->dirty = false;
     return 0;
 }
@@ -1040,17 +640,14 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
 }
 
 static void batch_launch(kite_batch *b, int pass, int mode, int step) {
-    const bool rows = pass == 0;
-    const int grid = rows ? b->nrows : b->ncols;
-    if (!grid) return;
     SlotDev none{};
-    const size_t lds = rows ? b->lds_rows : b->lds_cols;
-    if (pass == 0)
-        hipLaunchKernelGGL(k_rows, dim3(grid), dim3(kNT), lds, b->stream, b->d_slots, none, b->d_rows, mode, step);
-    else if (pass == 1)
-        hipLaunchKernelGGL(k_cols_inv, dim3(grid), dim3(kNT), lds, b->stream, b->d_slots, none, b->d_cols, mode, step);
-    else
-        hipLaunchKernelGGL(k_cols_fwd, dim3(grid), dim3(kNT), lds, b->stream, b->d_slots, none, b->d_cols, mode, step);
+    const auto &groups = pass == 0 ? b->grows : b->gcols;
+    const int2 *jobs = pass == 0 ? b->d_rows : b->d_cols;
+    for (const auto &g : groups) {
+        if (!g.count) continue;
+        const PassKernel fn = pass == 2 ? g.k2 : g.k1;
+        launch_pass(fn, g.count, g.lds, b->stream, b->d_slots, none, jobs + g.off, mode, step);
+    }
 }
 
 extern "C" kite_batch *kite_batch_new(void) {

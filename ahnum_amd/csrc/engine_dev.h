@@ -14,6 +14,43 @@ constexpr uint32_t kMaxR = 1024;    // column length
 constexpr uint32_t kMaxColLds = 96 * 1024;
 constexpr int kMaxLds = 160 * 1024; // gfx950 LDS per work-group
 constexpr int kMaxTop = 64;         // top-region digits handled by the device fold
+constexpr int kSub = 8;             // elements (16 digits) per carry sub-chunk
+constexpr int kTwLdsMaxC = 1024;    // row twiddles are staged in LDS up to this row length
+
+// Radix plan of a length-n FFT: 8s first, then 4, 2, 3, 5, 7 (host make_plan uses the same order).
+__host__ __device__ constexpr int ct_nstages(int n) {
+    int c = 0;
+    while (n % 8 == 0) { n /= 8; c++; }
+    const int pr[5] = {4, 2, 3, 5, 7};
+    for (int i = 0; i < 5; i++)
+        while (n % pr[i] == 0) { n /= pr[i]; c++; }
+    return c;
+}
+__host__ __device__ constexpr int ct_radix(int n, int idx) {
+    int c = 0;
+    while (n % 8 == 0) {
+        if (c == idx) return 8;
+        n /= 8;
+        c++;
+    }
+    const int pr[5] = {4, 2, 3, 5, 7};
+    for (int i = 0; i < 5; i++)
+        while (n % pr[i] == 0) {
+            if (c == idx) return pr[i];
+            n /= pr[i];
+            c++;
+        }
+    return 0;
+}
+// Column group width for column length R: about 2048 elements per work-group, 4..64 columns.
+__host__ __device__ constexpr int g_for_R(int R) {
+    int G = 64;
+    while (G > 4 && G * R > 2048) G /= 2;
+    return G;
+}
+// Row pairs per work-group for row length C in the compile-time row kernels (~2048 elements, so
+// that every radix-8 stage keeps all threads busy).
+__host__ __device__ constexpr int ppw_for_C(int C) { return C >= 1024 ? 1 : 1024 / C; }
 
 enum : int { kModeSquare = 0, kModeMul = 1, kModeDigits = 2, kModeRowFft = 3 };
 
@@ -30,6 +67,7 @@ struct ShapeDev {  // one (R, C, G) shape: H = R*C complex, NG = C/G column grou
     const double2 *twC;    // w_C^t, t < C
     const double2 *tlo;    // w_H^t, t < 2^sshift
     const double2 *thi;    // w_H^(t 2^sshift)
+    const double2 *twH;    // w_H^t, t < H (full table, used by the compile-time kernels)
 };
 
 struct SlotDev {  // one number being operated on (one test in a batch)

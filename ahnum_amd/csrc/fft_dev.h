@@ -1,0 +1,917 @@
+// Device code of the gfx950 squaring engine: FP64 butterflies, Stockham FFTs in LDS, the
+// real-split square/multiply, the balanced carry with the top fold, and the three pass kernels
+// as templates over the shape. A template argument of 0 means "read the size from the shape at
+// run time" (the generic kernels); a non-zero size makes every loop bound, stride, division and
+// radix a compile-time constant (the fast kernels, instantiated in kernels_fast.hip).
+//
+//   rows pass      per row pair (k_r, R-k_r): row FFT (length C) -> real-split square or
+//                  multiply -> inverse row FFT   (the reference's pass 2: src/neon.c:987-1054)
+//   cols_inv pass  per column group: conj four-step twiddle -> inverse column FFT (length R) ->
+//                  scale k/(4H w_j), round, max round-off, int64 carry over 16-digit sub-chunks,
+//                  the 128-bit top fold, absorption of each sub-chunk's carry by the next one of
+//                  the row; writes int32 digits and one carry per row chunk
+//   cols_fwd pass  per column group: absorb the previous row chunk's carry into the first
+//                  sub-chunk (its last digit keeps any remainder), weight, forward column FFT,
+//                  four-step twiddle   (together the reference's pass 1: src/neon.c:811-877)
+#ifndef KITE_FFT_DEV_H
+#define KITE_FFT_DEV_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine_dev.h"
+
+namespace kite {
+
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <bool INV> __device__ __forceinline__ double2 mul_mi(double2 a) {  // * -i (fwd) or * +i (inv)
+    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// ---- small DFTs, y_k = sum_j x_j e^{-+2 pi i jk/p} ----------------------------------------------
+
+template <int P, bool INV> struct Dft;
+
+template <bool INV> struct Dft<2, INV> {
+    static __device__ __forceinline__ void run(double2 *v) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV> struct Dft<4, INV> {
+    static __device__ __forceinline__ void run(double2 *v) {
+        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        double2 s13 = cadd(v[1], v[3]), d13 = mul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    }
+};
+
+template <bool INV> struct Dft<8, INV> {
+    static __device__ __forceinline__ void run(double2 *v) {
+        const double h = 0.70710678118654752440;
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV>::run(e);
+        Dft<4, INV>::run(o);
+        // o_k *= w8^k, w8 = e^{-+ i pi/4}
+        double2 o1 = INV ? make_double2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                         : make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+        double2 o2 = mul_mi<INV>(o[2]);
+        double2 o3 = INV ? make_double2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                         : make_double2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+template <int P> struct OddRoots;
+template <> struct OddRoots<3> {
+    static __device__ __forceinline__ double c(int) { return -0.5; }
+    static __device__ __forceinline__ double s(int q) { return q == 1 ? 0.86602540378443864676 : -0.86602540378443864676; }
+};
+template <> struct OddRoots<5> {
+    static __device__ __forceinline__ double c(int q) {
+        return (q == 1 || q == 4) ? 0.30901699437494742410 : -0.80901699437494742410;
+    }
+    static __device__ __forceinline__ double s(int q) {
+        return q == 1 ? 0.95105651629515357212 : q == 2 ? 0.58778525229247312917
+             : q == 3 ? -0.58778525229247312917 : -0.95105651629515357212;
+    }
+};
+template <> struct OddRoots<7> {
+    static __device__ __forceinline__ double c(int q) {
+        return (q == 1 || q == 6) ? 0.62348980185873353053
+             : (q == 2 || q == 5) ? -0.22252093395631440429 : -0.90096886790241912624;
+    }
+    static __device__ __forceinline__ double s(int q) {
+        return q == 1 ? 0.78183148246802980871 : q == 2 ? 0.97492791218182360702
+             : q == 3 ? 0.43388373911755812048 : q == 4 ? -0.43388373911755812048
+             : q == 5 ? -0.97492791218182360702 : -0.78183148246802980871;
+    }
+};
+
+// odd p: pairs (x_j, x_{p-j}) as s = sum, d = difference; y_k = A -+ iB, y_{p-k} = A +- iB
+template <int P, bool INV> struct DftOdd {
+    static __device__ __forceinline__ void run(double2 *v) {
+        constexpr int M = (P - 1) / 2;
+        double2 sp[M], dm[M];
+        double2 y0 = v[0];
+#pragma unroll
+        for (int j = 1; j <= M; j++) {
+            sp[j - 1] = cadd(v[j], v[P - j]);
+            dm[j - 1] = csub(v[j], v[P - j]);
+            y0 = cadd(y0, sp[j - 1]);
+        }
+        double2 out[P];
+        out[0] = y0;
+#pragma unroll
+        for (int k = 1; k <= M; k++) {
+            double2 A = v[0], B = make_double2(0, 0);
+#pragma unroll
+            for (int j = 1; j <= M; j++) {
+                const int q = (j * k) % P;
+                const double cq = OddRoots<P>::c(q), sq = OddRoots<P>::s(q);
+                A.x += sp[j - 1].x * cq;
+                A.y += sp[j - 1].y * cq;
+                B.x += dm[j - 1].x * sq;
+                B.y += dm[j - 1].y * sq;
+            }
+            double2 miB = make_double2(B.y, -B.x);
+            out[k] = INV ? csub(A, miB) : cadd(A, miB);
+            out[P - k] = INV ? cadd(A, miB) : csub(A, miB);
+        }
+#pragma unroll
+        for (int k = 0; k < P; k++) v[k] = out[k];
+    }
+};
+template <bool INV> struct Dft<3, INV> : DftOdd<3, INV> {};
+template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
+template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
+
+// ---- Stockham stages in LDS -----------------------------------------------------------------
+// `how` sequences of length n; element (seq, idx) at buf[idx*is + seq*ss]. A stage of radix P
+// after Ns points have been combined: butterfly j reads idx j + r*n/P, twiddles input r by
+// w_{Ns P}^{(j mod Ns) r} = tw[(j mod Ns) r n/(Ns P)], writes idx (j/Ns) Ns P + (j mod Ns) + r Ns.
+
+template <int P, bool INV>
+__device__ __forceinline__ void butterfly(const double2 *__restrict__ src, double2 *__restrict__ dst, int j,
+                                          int seq, int n, int Ns, int is, int ss, const double2 *__restrict__ tw) {
+    const int nb = n / P, ts = n / (Ns * P), jm = j % Ns;
+    double2 v[P];
+#pragma unroll
+    for (int r = 0; r < P; r++) v[r] = src[(j + r * nb) * is + seq * ss];
+    if (jm) {
+#pragma unroll
+        for (int r = 1; r < P; r++) {
+            const double2 t = tw[jm * r * ts];
+            v[r] = INV ? cmulc(v[r], t) : cmul(v[r], t);
+        }
+    }
+    Dft<P, INV>::run(v);
+    const int base = (j / Ns) * Ns * P + jm;
+#pragma unroll
+    for (int r = 0; r < P; r++) dst[(base + r * Ns) * is + seq * ss] = v[r];
+}
+
+// run-time plan
+template <int P, bool INV>
+__device__ __forceinline__ void stage_rt(const double2 *src, double2 *dst, int n, int Ns, int how, int is, int ss,
+                                         bool seqfast, const double2 *tw) {
+    const int nb = n / P, total = nb * how;
+    for (int w = threadIdx.x; w < total; w += kNT) {
+        const int seq = seqfast ? w % how : w / nb, j = seqfast ? w / how : w % nb;
+        butterfly<P, INV>(src, dst, j, seq, n, Ns, is, ss, tw);
+    }
+}
+
+template <bool INV>
+__device__ double2 *fft_rt(double2 *a, double2 *b, const Plan &pl, int how, int is, int ss, bool seqfast,
+                           const double2 *tw) {
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; s++) {
+        const int p = pl.rad[s];
+        switch (p) {
+        case 2: stage_rt<2, INV>(a, b, pl.n, Ns, how, is, ss, seqfast, tw); break;
+        case 3: stage_rt<3, INV>(a, b, pl.n, Ns, how, is, ss, seqfast, tw); break;
+        case 4: stage_rt<4, INV>(a, b, pl.n, Ns, how, is, ss, seqfast, tw); break;
+        case 5: stage_rt<5, INV>(a, b, pl.n, Ns, how, is, ss, seqfast, tw); break;
+        case 7: stage_rt<7, INV>(a, b, pl.n, Ns, how, is, ss, seqfast, tw); break;
+        default: stage_rt<8, INV>(a, b, pl.n, Ns, how, is, ss, seqfast, tw); break;
+        }
+        __syncthreads();
+        double2 *t = a;
+        a = b;
+        b = t;
+        Ns *= p;
+    }
+    return a;
+}
+
+// compile-time plan: the same radix order as the host's make_plan (8s, then 4, 2, 3, 5, 7)
+template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV>
+__device__ __forceinline__ double2 *fft_ct(double2 *a, double2 *b, const double2 *tw) {
+    if constexpr (I == ct_nstages(N)) {
+        return a;
+    } else {
+        constexpr int P = ct_radix(N, I), NB = N / P, TOTAL = NB * HOW;
+#pragma unroll
+        for (int w0 = 0; w0 < TOTAL; w0 += kNT) {
+            const int w = w0 + (int)threadIdx.x;
+            if (TOTAL % kNT == 0 || w < TOTAL) {
+                const int seq = SEQFAST ? w % HOW : w / NB, j = SEQFAST ? w / HOW : w % NB;
+                butterfly<P, INV>(a, b, j, seq, N, NS, IS, SS, tw);
+            }
+        }
+        __syncthreads();
+        return fft_ct<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV>(b, a, tw);
+    }
+}
+
+// w_H^t for t < H from the two-level table
+__device__ __forceinline__ double2 tw_h(const ShapeDev &sh, int t) {
+    return cmul(sh.thi[t >> sh.sshift], sh.tlo[t & ((1 << sh.sshift) - 1)]);
+}
+
+// ---- real-split pointwise square / multiply ----------------------------------------------------
+// a = U_k, b = U_{H-k}, t = w_H^k; e = a + conj(b), o = -i (a - conj(b)).
+// square: X = e^2 + t o^2, Y = 2 e o; multiply by the operand's (ya, yb): X = e ey + t o oy,
+// Y = e oy + o ey. New a = X + iY, new b = conj(X) + i conj(Y) (4x the packed product's DFT).
+__device__ __forceinline__ void split_sq(double2 &a, double2 &b, double2 t) {
+    double2 e = make_double2(a.x + b.x, a.y - b.y);
+    double2 o = make_double2(a.y + b.y, b.x - a.x);
+    double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2 * e.x * e.y);
+    double2 o2 = make_double2(o.x * o.x - o.y * o.y, 2 * o.x * o.y);
+    double2 X = cadd(e2, cmul(t, o2));
+    double2 Y = make_double2(2 * (e.x * o.x - e.y * o.y), 2 * (e.x * o.y + e.y * o.x));
+    a = make_double2(X.x - Y.y, X.y + Y.x);
+    b = make_double2(X.x + Y.y, Y.x - X.y);
+}
+
+__device__ __forceinline__ void split_mul(double2 &a, double2 &b, double2 t, double2 ya, double2 yb) {
+    double2 e = make_double2(a.x + b.x, a.y - b.y);
+    double2 o = make_double2(a.y + b.y, b.x - a.x);
+    double2 ey = make_double2(ya.x + yb.x, ya.y - yb.y);
+    double2 oy = make_double2(ya.y + yb.y, yb.x - ya.x);
+    double2 X = cadd(cmul(e, ey), cmul(t, cmul(o, oy)));
+    double2 Y = cadd(cmul(e, oy), cmul(o, ey));
+    a = make_double2(X.x - Y.y, X.y + Y.x);
+    b = make_double2(X.x + Y.y, Y.x - X.y);
+}
+
+__device__ __forceinline__ const SlotDev &pick_slot(const SlotDev *slots, const SlotDev &one, int i) {
+    return slots ? slots[i] : one;
+}
+
+// floor(h / k) for 0 < k < 2^32
+__device__ __forceinline__ i128 floordiv_small(i128 h, uint32_t k) {
+    const bool neg = h < 0;
+    u128 a = neg ? (u128)(-h) : (u128)h;
+    uint64_t hi = (uint64_t)(a >> 64), lo = (uint64_t)a;
+    uint64_t qh = hi / k, r = hi % k;
+    uint64_t t = (r << 32) | (lo >> 32);
+    uint64_t q1 = t / k;
+    r = t % k;
+    t = (r << 32) | (lo & 0xffffffffull);
+    uint64_t q0 = t / k;
+    r = t % k;
+    u128 q = ((u128)qh << 64) | ((u128)q1 << 32) | q0;
+    if (!neg) return (i128)q;
+    return -(i128)q - (r ? 1 : 0);
+}
+
+// Fold the top region (digits top_t..L-1, within the last sub-chunk, plus the carry at 2^{s_L}):
+// H = q*k + r  ->  q into digit 0, r*2^m rebalanced into the top digits (src/ctx.c:345-375).
+__device__ __forceinline__ int64_t fold_top_dev(const SlotDev &s, int32_t *d, int64_t carry) {
+    const int t0 = s.top_i0, nt = s.top_n, m = s.top_m;
+    i128 Hh = (i128)carry << s.top_shift;
+    int pos = 0;
+    for (int i = 0; i < nt; i++) {
+        Hh += (i128)d[t0 + i] << pos;
+        pos += s.top_bits[i];
+    }
+    const i128 K = (i128)s.k;
+    const i128 lo = Hh & (((i128)1 << m) - 1), hi = Hh >> m;
+    i128 q = floordiv_small(hi, (uint32_t)s.k);
+    i128 top = lo + ((hi - q * K) << m);
+    if (2 * top >= (K << m)) {
+        top -= K << m;
+        q += 1;
+    }
+    for (int i = 0; i + 1 < nt; i++) {
+        const int b = s.top_bits[i];
+        const i128 qq = (top + ((i128)1 << (b - 1))) >> b;
+        d[t0 + i] = (int32_t)(top - (qq << b));
+        top = qq;
+    }
+    if (nt > 0) d[t0 + nt - 1] = (int32_t)top;
+    return (int64_t)q;
+}
+
+__device__ __forceinline__ double pow2i(int e) { return e == 0 ? 1.0 : e == 1 ? 2.0 : 4.0; }
+__device__ __forceinline__ double pow2ni(int e) { return e == 0 ? 1.0 : e == 1 ? 0.5 : 0.25; }
+
+// Balanced split in exact FP64 arithmetic: x is an integer with |x| < 2^52, so x + 2^(b-1), the
+// scaling by 2^-b, floor and x - q 2^b are all exact. Returns the carry q, sets the digit d.
+__device__ __forceinline__ double bal_f(double x, double base, double ibase, double half, double &d) {
+    const double q = floor((x + half) * ibase);
+    d = fma(-q, base, x);
+    return q;
+}
+
+// balanced split of v into a digit of b bits and the carry
+__device__ __forceinline__ int64_t bal(int64_t v, int b, int32_t &d) {
+    const int64_t q = (v + ((int64_t)1 << (b - 1))) >> b;
+    d = (int32_t)(v - (q << b));
+    return q;
+}
+
+// ---- rows pass ---------------------------------------------------------------------------------
+// Row job u covers pairs [u*PPW, (u+1)*PPW): pair 0 is (0, R/2) (both self-paired; R/2 absent
+// for odd R), pair p > 0 is (p, R-p). LDS holds 2*PPW rows of C (row slot 2i+h) twice.
+
+template <int CC>
+__device__ __forceinline__ void rows_body(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+                                          int step) {
+    extern __shared__ double2 lds[];
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
+    const ShapeDev &sh = *s.shp;
+    constexpr int PPW_CT = CC ? ppw_for_C(CC) : 1;
+    const int R = sh.R, C = CC ? CC : sh.C, PPW = PPW_CT, NROW = 2 * PPW;
+    const int npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
+    double2 *A = lds, *B = lds + NROW * C;
+    const double2 *twC = sh.twC;
+    if (CC && CC <= kTwLdsMaxC) {
+        double2 *T = lds + 2 * NROW * C;
+        for (int i = threadIdx.x; i < C; i += kNT) T[i] = sh.twC[i];
+        twC = T;
+    }
+    const double2 *x = mode == kModeRowFft ? s.y_src : s.x;
+    // row index of slot (i, h), or -1
+    auto row_of = [&](int i, int h) -> int {
+        const int p = jb.y * PPW + i;
+        if (p >= npairs) return -1;
+        if (p == 0) return h == 0 ? 0 : (R % 2 == 0 && R > 1 ? R / 2 : -1);
+        return h == 0 ? p : R - p;
+    };
+    for (int e = threadIdx.x; e < NROW * C; e += kNT) {
+        const int slot = e / C, c = e - slot * C;
+        const int row = row_of(slot >> 1, slot & 1);
+        A[e] = row >= 0 ? x[(size_t)row * C + c] : make_double2(0, 0);
+    }
+    __syncthreads();
+    double2 *F;
+    if constexpr (CC != 0) F = fft_ct<CC, 1, 0, 2 * PPW_CT, 1, CC, false, false>(A, B, twC);
+    else F = fft_rt<false>(A, B, sh.pc, NROW, 1, C, false, twC);
+    if (mode == kModeRowFft) {  // operand preparation: store the row spectra
+        for (int e = threadIdx.x; e < NROW * C; e += kNT) {
+            const int slot = e / C, c = e - slot * C;
+            const int row = row_of(slot >> 1, slot & 1);
+            if (row >= 0) s.yfreq[(size_t)row * C + c] = F[e];
+        }
+        return;
+    }
+    const bool mul = mode == kModeMul;
+    const double2 *Y = s.yfreq;
+    for (int w = threadIdx.x; w < NROW * C; w += kNT) {
+        const int slot = w / C, kc = w - slot * C, i = slot >> 1, h = slot & 1;
+        const int p = jb.y * PPW + i;
+        if (p >= npairs) continue;
+        int ka, kb, pa, pb;  // rows and columns of the pair's two elements
+        if (p == 0) {
+            ka = row_of(0, h);
+            if (ka < 0) continue;
+            kb = ka;
+            pa = kc;
+            pb = h == 0 ? (C - kc) % C : C - 1 - kc;
+            if (pb < kc) continue;
+        } else {
+            if (h) continue;
+            ka = p;
+            kb = R - p;
+            pa = kc;
+            pb = C - 1 - kc;
+        }
+        double2 *ea = &F[slot * C + pa];
+        double2 *eb = &F[(p == 0 ? slot : slot + 1) * C + pb];
+        double2 a = *ea, b = *eb;
+        const double2 t = tw_h(sh, ka + R * kc);
+        if (mul) split_mul(a, b, t, Y[(size_t)ka * C + pa], Y[(size_t)kb * C + pb]);
+        else split_sq(a, b, t);
+        *ea = a;
+        if (eb != ea) *eb = b;
+    }
+    __syncthreads();
+    double2 *Gb;
+    if constexpr (CC != 0) Gb = fft_ct<CC, 1, 0, 2 * PPW_CT, 1, CC, false, true>(F, F == A ? B : A, twC);
+    else Gb = fft_rt<true>(F, F == A ? B : A, sh.pc, NROW, 1, C, false, twC);
+    for (int e = threadIdx.x; e < NROW * C; e += kNT) {
+        const int slot = e / C, c = e - slot * C;
+        const int row = row_of(slot >> 1, slot & 1);
+        if (row >= 0) s.x[(size_t)row * C + c] = Gb[e];
+    }
+}
+
+// ---- column passes -----------------------------------------------------------------------------
+// LDS: A, B (R*G complex each), then the column twiddles (R complex) and one int64 carry per
+// sub-chunk. Digits are staged as int32 [r][2G] in whichever of A/B is free.
+
+template <int RC>
+__device__ __forceinline__ void cols_inv_body(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+                                              int step) {
+    extern __shared__ double2 lds[];
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;
+    const ShapeDev &sh = *s.shp;
+    constexpr int G_CT = RC ? g_for_R(RC) : 0;
+    const int R = RC ? RC : sh.R, G = RC ? G_CT : sh.G, C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    const int SUB = G < kSub ? G : kSub, NSUB = G / SUB;
+    double2 *A = lds, *B = lds + R * G, *T = lds + 2 * R * G;
+    int64_t *SC = reinterpret_cast<int64_t *>(T + R);
+    const double2 *twR = sh.twR;
+    if (RC) {
+        for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+        twR = T;
+    }
+    for (int i = threadIdx.x; i < R * G; i += kNT) {
+        const int kr = i / G, l = i - kr * G;
+        A[i] = cmulc(s.x[(size_t)kr * C + c0 + l], tw_h(sh, (c0 + l) * kr));
+    }
+    __syncthreads();
+    double2 *F;
+    if constexpr (RC != 0) F = fft_ct<RC, 1, 0, G_CT, G_CT, 1, true, true>(A, B, twR);
+    else F = fft_rt<true>(A, B, sh.pr, G, G, 1, true, twR);
+    int32_t *D = reinterpret_cast<int32_t *>(F == A ? B : A);
+    const bool sq = mode != kModeDigits;
+    const double sc0 = sq ? s.kscale : s.rscale;
+    const uint8_t *code = s.dcode + (size_t)g * R * G;
+    double lmax = 0;
+    const double bb = (double)((int64_t)1 << s.b0), ib = 1.0 / bb;
+    for (int w = threadIdx.x; w < R * NSUB; w += kNT) {
+        const int r = w / NSUB, q = w - r * NSUB;
+        const double tap = s.ta[R + r] * sc0;
+        // all loads and the rounding first (independent), then the serial carry in registers
+        double rx[2 * kSub];
+        int bigs[2 * kSub];
+#pragma unroll
+        for (int l = 0; l < kSub; l++) {
+            if (l >= SUB) break;
+            const int col = q * SUB + l;
+            const double2 v = F[r * G + col];
+            const int cd = code[r * G + col];
+            const int cb = 2 * (c0 + col);
+            const double x0 = v.x * (tap * s.tb[2 * C + cb] * pow2ni(cd & 3));
+            const double x1 = v.y * (tap * s.tb[2 * C + cb + 1] * pow2ni((cd >> 3) & 3));
+            rx[2 * l] = rint(x0);
+            rx[2 * l + 1] = rint(x1);
+            lmax = fmax(lmax, fmax(fabs(x0 - rx[2 * l]), fabs(x1 - rx[2 * l + 1])));
+            bigs[2 * l] = (cd >> 2) & 1;
+            bigs[2 * l + 1] = (cd >> 5) & 1;
+        }
+        int32_t *dr = D + r * 2 * G + 2 * q * SUB;
+        double cy = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * kSub; i++) {
+            if (i >= 2 * SUB) break;
+            if (sq) {
+                const double base = bigs[i] ? 2 * bb : bb, ibase = bigs[i] ? 0.5 * ib : ib;
+                double d;
+                cy = bal_f(rx[i] + cy, base, ibase, 0.5 * base, d);
+                dr[i] = (int32_t)d;
+            } else {
+                dr[i] = (int32_t)rx[i];
+            }
+        }
+        SC[w] = (int64_t)cy;
+    }
+    __syncthreads();
+    if (sq) {
+        if (g == NG - 1 && threadIdx.x == 0) {  // the top fold on the number's last sub-chunk
+            const int w = R * NSUB - 1;
+            SC[w] = fold_top_dev(s, D + (R - 1) * 2 * G + 2 * (NSUB - 1) * SUB, SC[w]);
+        }
+        __syncthreads();
+        // each sub-chunk after the first of its row absorbs its left neighbour's carry
+        for (int w = threadIdx.x; w < R * NSUB; w += kNT) {
+            const int r = w / NSUB, q = w - r * NSUB;
+            if (q == 0) continue;
+            int64_t cy = SC[w - 1];
+            int32_t *dr = D + r * 2 * G + 2 * q * SUB;
+            for (int i = 0; i < 2 * SUB && cy; i++) {
+                if (i == 2 * SUB - 1) {
+                    dr[i] = (int32_t)(dr[i] + cy);
+                    break;
+                }
+                const int cd = code[r * G + q * SUB + (i >> 1)];
+                const int big = (i & 1) ? (cd >> 5) & 1 : (cd >> 2) & 1;
+                cy = bal((int64_t)dr[i] + cy, s.b0 + big, dr[i]);
+            }
+        }
+        for (int r = threadIdx.x; r < R; r += kNT) s.cout[g * R + r] = SC[r * NSUB + NSUB - 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+    if ((threadIdx.x & 63) == 0 && lmax > 0)
+        atomicMax(s.maxerr, (unsigned long long)__double_as_longlong(lmax));
+    __syncthreads();
+    int32_t *out = s.dig + (size_t)g * R * 2 * G;
+    for (int i = threadIdx.x; i < R * 2 * G; i += kNT) out[i] = D[i];
+}
+
+template <int RC>
+__device__ __forceinline__ void cols_fwd_body(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+                                              int step) {
+    extern __shared__ double2 lds[];
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;
+    const ShapeDev &sh = *s.shp;
+    constexpr int G_CT = RC ? g_for_R(RC) : 0;
+    const int R = RC ? RC : sh.R, G = RC ? G_CT : sh.G, C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    const int SUB = G < kSub ? G : kSub;
+    double2 *A = lds, *B = lds + R * G, *T = lds + 2 * R * G;
+    const double2 *twR = sh.twR;
+    if (RC) {
+        for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+        twR = T;
+    }
+    int32_t *D = reinterpret_cast<int32_t *>(B);
+    const int32_t *in = s.dig + (size_t)g * R * 2 * G;
+    for (int i = threadIdx.x; i < R * 2 * G; i += kNT) D[i] = in[i];
+    __syncthreads();
+    const uint8_t *code = s.dcode + (size_t)g * R * G;
+    if (mode != kModeDigits) {
+        for (int r = threadIdx.x; r < R; r += kNT) {
+            int pidx;  // row chunk before (g, r) in digit order, cyclically
+            if (g > 0) pidx = (g - 1) * R + r;
+            else if (r > 0) pidx = (NG - 1) * R + r - 1;
+            else pidx = (NG - 1) * R + R - 1;
+            int64_t cy = s.cout[pidx];
+            int32_t *dr = D + r * 2 * G;
+            for (int i = 0; i < 2 * SUB && cy; i++) {
+                if (i == 2 * SUB - 1) {
+                    dr[i] = (int32_t)(dr[i] + cy);
+                    break;
+                }
+                const int cd = code[r * G + (i >> 1)];
+                const int big = (i & 1) ? (cd >> 5) & 1 : (cd >> 2) & 1;
+                cy = bal((int64_t)dr[i] + cy, s.b0 + big, dr[i]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < R * G; i += kNT) {
+        const int r = i / G, l = i - r * G;
+        const int cd = code[i];
+        const double tam = s.ta[r];
+        const int cb = 2 * (c0 + l);
+        const double w0 = tam * s.tb[cb] * pow2i(cd & 3);
+        const double w1 = tam * s.tb[cb + 1] * pow2i((cd >> 3) & 3);
+        A[i] = make_double2((double)D[r * 2 * G + 2 * l] * w0, (double)D[r * 2 * G + 2 * l + 1] * w1);
+    }
+    __syncthreads();
+    double2 *F;
+    if constexpr (RC != 0) F = fft_ct<RC, 1, 0, G_CT, G_CT, 1, true, false>(A, B, twR);
+    else F = fft_rt<false>(A, B, sh.pr, G, G, 1, true, twR);
+    for (int i = threadIdx.x; i < R * G; i += kNT) {
+        const int kr = i / G, l = i - kr * G;
+        s.x[(size_t)kr * C + c0 + l] = cmul(F[i], tw_h(sh, (c0 + l) * kr));
+    }
+}
+
+// ================================================================================================
+// Compile-time kernels: single-buffer Stockham stages (a stage's inputs are held in registers
+// across one barrier, halving LDS per work-group), one thread per 16-digit carry sub-chunk working
+// in registers and moving its digits straight between HBM and registers, the full w_H table.
+// ================================================================================================
+
+template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV>
+__device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
+    if constexpr (I < ct_nstages(N)) {
+        constexpr int P = ct_radix(N, I), NB = N / P, TOTAL = NB * HOW;
+        constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
+        double2 v[ITER][P];
+        int ob[ITER];
+#pragma unroll
+        for (int it = 0; it < ITER; it++) {
+            const int w = it * kNT + (int)threadIdx.x;
+            if (TOTAL % kNT == 0 || w < TOTAL) {
+                const int seq = SEQFAST ? w % HOW : w / NB, j = SEQFAST ? w / HOW : w % NB;
+                const int jm = j % NS;
+#pragma unroll
+                for (int r = 0; r < P; r++) v[it][r] = a[(j + r * NB) * IS + seq * SS];
+                if (NS > 1 && jm) {
+#pragma unroll
+                    for (int r = 1; r < P; r++) {
+                        const double2 t = tw[jm * r * TS];
+                        v[it][r] = INV ? cmulc(v[it][r], t) : cmul(v[it][r], t);
+                    }
+                }
+                Dft<P, INV>::run(v[it]);
+                ob[it] = ((j / NS) * NS * P + jm) * IS + seq * SS;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < ITER; it++) {
+            const int w = it * kNT + (int)threadIdx.x;
+            if (TOTAL % kNT == 0 || w < TOTAL) {
+#pragma unroll
+                for (int r = 0; r < P; r++) a[ob[it] + r * NS * IS] = v[it][r];
+            }
+        }
+        __syncthreads();
+        fft_ip<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV>(a, tw);
+    }
+}
+
+// Rows: 2*PPW rows of C in LDS (row slot 2i+h = side h of local pair i), twiddles w_C after them.
+template <int C>
+__device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+                                          int step) {
+    extern __shared__ double2 lds[];
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;
+    const ShapeDev &sh = *s.shp;
+    constexpr int PPW = ppw_for_C(C), NROW = 2 * PPW, NE = NROW * C;
+    constexpr bool TWL = C <= kTwLdsMaxC;
+    const int R = sh.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
+    double2 *A = lds;
+    const double2 *twC = sh.twC;
+    if (TWL) {
+        double2 *T = lds + NE;
+        for (int i = threadIdx.x; i < C; i += kNT) T[i] = sh.twC[i];
+        twC = T;
+    }
+    auto row_of = [&](int i, int h) -> int {
+        const int p = jb.y * PPW + i;
+        if (p >= npairs) return -1;
+        if (p == 0) return h == 0 ? 0 : (R % 2 == 0 && R > 1 ? R / 2 : -1);
+        return h == 0 ? p : R - p;
+    };
+    const double2 *x = mode == kModeRowFft ? s.y_src : s.x;
+    for (int e = threadIdx.x; e < NE; e += kNT) {
+        const int slot = e / C, c = e - slot * C;
+        const int row = row_of(slot >> 1, slot & 1);
+        A[e] = row >= 0 ? x[(size_t)row * C + c] : make_double2(0, 0);
+    }
+    // split items: one per (local pair, k_c); the twiddle of item kc of pair p > 0 (and of row 0 of
+    // pair 0) is fetched before the FFT
+    constexpr int NIT = PPW * C, ITEMS = (NIT + kNT - 1) / kNT;
+    const bool split = mode == kModeSquare || mode == kModeMul;
+    double2 tw_[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        const int i = w / C, kc = w - i * C, p = jb.y * PPW + i;
+        tw_[it] = (split && (NIT % kNT == 0 || w < NIT) && p < npairs) ? sh.twH[(p == 0 ? 0 : p) + R * kc]
+                                                                          : make_double2(1, 0);
+    }
+    __syncthreads();
+    fft_ip<C, 1, 0, NROW, 1, C, false, false>(A, twC);
+    if (mode == kModeRowFft) {  // operand preparation: store the row spectra
+        for (int e = threadIdx.x; e < NE; e += kNT) {
+            const int slot = e / C, c = e - slot * C;
+            const int row = row_of(slot >> 1, slot & 1);
+            if (row >= 0) s.yfreq[(size_t)row * C + c] = A[e];
+        }
+        return;
+    }
+    const bool mul = mode == kModeMul;
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        if (NIT % kNT != 0 && w >= NIT) continue;
+        const int i = w / C, kc = w - i * C, p = jb.y * PPW + i;
+        if (p >= npairs) continue;
+        if (p > 0) {  // (p, kc) with (R-p, C-1-kc)
+            const int pb = C - 1 - kc;
+            double2 a = A[2 * i * C + kc], b = A[(2 * i + 1) * C + pb];
+            if (mul) split_mul(a, b, tw_[it], s.yfreq[(size_t)p * C + kc], s.yfreq[(size_t)(R - p) * C + pb]);
+            else split_sq(a, b, tw_[it]);
+            A[2 * i * C + kc] = a;
+            A[(2 * i + 1) * C + pb] = b;
+        } else {
+            // row 0: kc with (C-kc) mod C; row R/2 (if present): kc with C-1-kc
+            int pb = (C - kc) % C;
+            if (pb >= kc) {
+                double2 a = A[kc], b = A[pb];
+                if (mul) split_mul(a, b, tw_[it], s.yfreq[kc], s.yfreq[pb]);
+                else split_sq(a, b, tw_[it]);
+                A[kc] = a;
+                if (pb != kc) A[pb] = b;
+            }
+            const int rh = row_of(0, 1);
+            pb = C - 1 - kc;
+            if (rh >= 0 && pb >= kc) {
+                const double2 t = sh.twH[rh + R * kc];
+                double2 a = A[C + kc], b = A[C + pb];
+                if (mul) split_mul(a, b, t, s.yfreq[(size_t)rh * C + kc], s.yfreq[(size_t)rh * C + pb]);
+                else split_sq(a, b, t);
+                A[C + kc] = a;
+                if (pb != kc) A[C + pb] = b;
+            }
+        }
+    }
+    __syncthreads();
+    fft_ip<C, 1, 0, NROW, 1, C, false, true>(A, twC);
+    for (int e = threadIdx.x; e < NE; e += kNT) {
+        const int slot = e / C, c = e - slot * C;
+        const int row = row_of(slot >> 1, slot & 1);
+        if (row >= 0) s.x[(size_t)row * C + c] = A[e];
+    }
+}
+
+// Column passes: LDS = R*G complex, the column twiddles (R complex), one int64 carry per
+// sub-chunk, and a 64-digit scratch for the top fold.
+template <int R> struct ColGeom {
+    static constexpr int G = g_for_R(R), SUB = G < kSub ? G : kSub, NSUB = G / SUB, NE = R * G;
+    static constexpr int NSC = R * NSUB, ITSC = (NSC + kNT - 1) / kNT;
+    static constexpr size_t lds_bytes() { return (size_t)16 * (NE + R) + (size_t)8 * NSC + 4 * 64; }
+};
+
+template <int R>
+__device__ __forceinline__ void cols_inv_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+                                              int step) {
+    extern __shared__ double2 lds[];
+    using Gm = ColGeom<R>;
+    constexpr int G = Gm::G, SUB = Gm::SUB, NSUB = Gm::NSUB, NE = Gm::NE, NSC = Gm::NSC, ITSC = Gm::ITSC;
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;
+    const ShapeDev &sh = *s.shp;
+    const int C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    double2 *A = lds, *T = lds + NE;
+    int64_t *SC = reinterpret_cast<int64_t *>(T + R);
+    int32_t *FD = reinterpret_cast<int32_t *>(SC + NSC);
+    for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+    for (int i = threadIdx.x; i < NE; i += kNT) {
+        const int kr = i / G, l = i - kr * G;
+        A[i] = cmulc(s.x[(size_t)kr * C + c0 + l], sh.twH[(c0 + l) * kr]);
+    }
+    __syncthreads();
+    fft_ip<R, 1, 0, G, G, 1, true, true>(A, T);
+    const bool sq = mode != kModeDigits;
+    const double sc0 = sq ? s.kscale : s.rscale;
+    const double bb = (double)((int64_t)1 << s.b0), ibb = 1.0 / bb;
+    const uint8_t *code = s.dcode + (size_t)g * R * G;
+    double lmax = 0;
+    int32_t d[ITSC][2 * SUB];
+    uint32_t bg[ITSC];  // bit i: digit i of the sub-chunk has b0+1 bits
+#pragma unroll
+    for (int it = 0; it < ITSC; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int r = w / NSUB, q = w - r * NSUB;
+        const double tap = s.ta[R + r] * sc0;
+        double cy = 0;
+        bg[it] = 0;
+#pragma unroll
+        for (int l = 0; l < SUB; l++) {
+            const int col = q * SUB + l;
+            const double2 v = A[r * G + col];
+            const int cd = code[r * G + col];
+            const int cb = 2 * (c0 + col);
+            const double x0 = v.x * (tap * s.tb[2 * C + cb] * pow2ni(cd & 3));
+            const double x1 = v.y * (tap * s.tb[2 * C + cb + 1] * pow2ni((cd >> 3) & 3));
+            const double r0 = rint(x0), r1 = rint(x1);
+            lmax = fmax(lmax, fmax(fabs(x0 - r0), fabs(x1 - r1)));
+            const int b0f = (cd >> 2) & 1, b1f = (cd >> 5) & 1;
+            bg[it] |= (uint32_t)(b0f | (b1f << 1)) << (2 * l);
+            if (sq) {
+                double dd;
+                cy = bal_f(r0 + cy, b0f ? 2 * bb : bb, b0f ? 0.5 * ibb : ibb, b0f ? bb : 0.5 * bb, dd);
+                d[it][2 * l] = (int32_t)dd;
+                cy = bal_f(r1 + cy, b1f ? 2 * bb : bb, b1f ? 0.5 * ibb : ibb, b1f ? bb : 0.5 * bb, dd);
+                d[it][2 * l + 1] = (int32_t)dd;
+            } else {
+                d[it][2 * l] = (int32_t)r0;
+                d[it][2 * l + 1] = (int32_t)r1;
+            }
+        }
+        int64_t co = (int64_t)cy;
+        if (sq && g == NG - 1 && w == NSC - 1) {  // the top fold on the number's last sub-chunk
+#pragma unroll
+            for (int i = 0; i < 2 * SUB; i++) FD[i] = d[it][i];
+            co = fold_top_dev(s, FD, co);
+#pragma unroll
+            for (int i = 0; i < 2 * SUB; i++) d[it][i] = FD[i];
+        }
+        SC[w] = co;
+    }
+    __syncthreads();
+    int32_t *out = s.dig + (size_t)g * R * 2 * G;
+#pragma unroll
+    for (int it = 0; it < ITSC; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int r = w / NSUB, q = w - r * NSUB;
+        if (sq && q > 0) {  // absorb the left neighbour's carry; the last digit keeps the rest
+            int64_t cy = SC[w - 1];
+#pragma unroll
+            for (int i = 0; i < 2 * SUB; i++) {
+                if (!cy) break;
+                if (i == 2 * SUB - 1) {
+                    d[it][i] = (int32_t)(d[it][i] + cy);
+                    break;
+                }
+                cy = bal((int64_t)d[it][i] + cy, s.b0 + (int)((bg[it] >> i) & 1), d[it][i]);
+            }
+        }
+        int4 *o4 = reinterpret_cast<int4 *>(out + r * 2 * G + 2 * q * SUB);
+#pragma unroll
+        for (int i = 0; i < 2 * SUB / 4; i++)
+            o4[i] = make_int4(d[it][4 * i], d[it][4 * i + 1], d[it][4 * i + 2], d[it][4 * i + 3]);
+        if (sq && q == NSUB - 1) s.cout[g * R + r] = SC[w];
+    }
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+    if ((threadIdx.x & 63) == 0 && lmax > 0)
+        atomicMax(s.maxerr, (unsigned long long)__double_as_longlong(lmax));
+}
+
+template <int R>
+__device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+                                              int step) {
+    extern __shared__ double2 lds[];
+    using Gm = ColGeom<R>;
+    constexpr int G = Gm::G, SUB = Gm::SUB, NSUB = Gm::NSUB, NE = Gm::NE, NSC = Gm::NSC, ITSC = Gm::ITSC;
+    constexpr int ITE = (NE + kNT - 1) / kNT;
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;
+    const ShapeDev &sh = *s.shp;
+    const int C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    double2 *A = lds, *T = lds + NE;
+    for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+    // four-step twiddles for the store, fetched early
+    double2 tws[ITE];
+#pragma unroll
+    for (int it = 0; it < ITE; it++) {
+        const int i = it * kNT + (int)threadIdx.x;
+        if (NE % kNT != 0 && i >= NE) continue;
+        const int kr = i / G, l = i - kr * G;
+        tws[it] = sh.twH[(c0 + l) * kr];
+    }
+    const int32_t *in = s.dig + (size_t)g * R * 2 * G;
+    const uint8_t *code = s.dcode + (size_t)g * R * G;
+    const bool sq = mode != kModeDigits;
+#pragma unroll
+    for (int it = 0; it < ITSC; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int r = w / NSUB, q = w - r * NSUB;
+        int32_t d[2 * SUB];
+        const int4 *i4 = reinterpret_cast<const int4 *>(in + r * 2 * G + 2 * q * SUB);
+#pragma unroll
+        for (int i = 0; i < 2 * SUB / 4; i++) {
+            const int4 v = i4[i];
+            d[4 * i] = v.x;
+            d[4 * i + 1] = v.y;
+            d[4 * i + 2] = v.z;
+            d[4 * i + 3] = v.w;
+        }
+        int cds[SUB];
+#pragma unroll
+        for (int l = 0; l < SUB; l++) cds[l] = code[r * G + q * SUB + l];
+        if (sq && q == 0) {  // the previous row chunk's carry, absorbed within this sub-chunk
+            int pidx;
+            if (g > 0) pidx = (g - 1) * R + r;
+            else if (r > 0) pidx = (NG - 1) * R + r - 1;
+            else pidx = (NG - 1) * R + R - 1;
+            int64_t cy = s.cout[pidx];
+#pragma unroll
+            for (int i = 0; i < 2 * SUB; i++) {
+                if (!cy) break;
+                if (i == 2 * SUB - 1) {
+                    d[i] = (int32_t)(d[i] + cy);
+                    break;
+                }
+                const int big = (i & 1) ? (cds[i >> 1] >> 5) & 1 : (cds[i >> 1] >> 2) & 1;
+                cy = bal((int64_t)d[i] + cy, s.b0 + big, d[i]);
+            }
+        }
+        const double tam = s.ta[r];
+#pragma unroll
+        for (int l = 0; l < SUB; l++) {
+            const int cb = 2 * (c0 + q * SUB + l);
+            const double w0 = tam * s.tb[cb] * pow2i(cds[l] & 3);
+            const double w1 = tam * s.tb[cb + 1] * pow2i((cds[l] >> 3) & 3);
+            A[r * G + q * SUB + l] = make_double2((double)d[2 * l] * w0, (double)d[2 * l + 1] * w1);
+        }
+    }
+    __syncthreads();
+    fft_ip<R, 1, 0, G, G, 1, true, false>(A, T);
+#pragma unroll
+    for (int it = 0; it < ITE; it++) {
+        const int i = it * kNT + (int)threadIdx.x;
+        if (NE % kNT != 0 && i >= NE) continue;
+        const int kr = i / G, l = i - kr * G;
+        s.x[(size_t)kr * C + c0 + l] = cmul(A[i], tws[it]);
+    }
+}
+
+}  // namespace kite
+
+#endif

@@ -1,0 +1,27 @@
+// Registry of the pass kernels (host side): generic kernels for any shape, compile-time-size
+// kernels for the preferred row lengths C and column lengths R.
+#ifndef KITE_KERNELS_H
+#define KITE_KERNELS_H
+
+#include <stddef.h>
+
+#include "engine_dev.h"
+
+namespace kite {
+
+typedef const void *PassKernel;  // a __global__ function (const SlotDev*, SlotDev, const int2*, int, int)
+
+PassKernel generic_rows();
+PassKernel generic_cols_inv();
+PassKernel generic_cols_fwd();
+// nullptr when no compile-time kernel exists for that size
+PassKernel fast_rows(int C);
+PassKernel fast_cols_inv(int R);
+PassKernel fast_cols_fwd(int R);
+size_t fast_cols_lds(int R);  // LDS bytes of the compile-time column kernels
+// every kernel pointer, for raising the LDS limit once
+void all_kernels(PassKernel *out, int *count, int cap);
+
+}  // namespace kite
+
+#endif

@@ -1,0 +1,144 @@
+// Compile-time-size pass kernels: row kernels for row lengths C = 64..2048 and column kernels for
+// column lengths R = 2^b m (m in 1, 3, 5, 7; 8 <= R <= 768). Generated list; the bodies are the
+// templates of fft_dev.h.
+#include "fft_dev.h"
+#include "kernels.h"
+
+namespace kite {
+
+template <int C>
+__global__ void __launch_bounds__(kNT) k_rows_c(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    rows_fast<C>(slots, one, jobs, mode, step);
+}
+template <int R>
+__global__ void __launch_bounds__(kNT) k_cols_inv_r(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    cols_inv_fast<R>(slots, one, jobs, mode, step);
+}
+template <int R>
+__global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    cols_fwd_fast<R>(slots, one, jobs, mode, step);
+}
+
+size_t fast_cols_lds(int R) {
+    switch (R) {
+#define KITE_COLS_LDS(r) case r: return ColGeom<r>::lds_bytes();
+    KITE_COLS_LDS(8)
+    KITE_COLS_LDS(12)
+    KITE_COLS_LDS(16)
+    KITE_COLS_LDS(20)
+    KITE_COLS_LDS(24)
+    KITE_COLS_LDS(28)
+    KITE_COLS_LDS(32)
+    KITE_COLS_LDS(40)
+    KITE_COLS_LDS(48)
+    KITE_COLS_LDS(56)
+    KITE_COLS_LDS(64)
+    KITE_COLS_LDS(80)
+    KITE_COLS_LDS(96)
+    KITE_COLS_LDS(112)
+    KITE_COLS_LDS(128)
+    KITE_COLS_LDS(160)
+    KITE_COLS_LDS(192)
+    KITE_COLS_LDS(224)
+    KITE_COLS_LDS(256)
+    KITE_COLS_LDS(320)
+    KITE_COLS_LDS(384)
+    KITE_COLS_LDS(448)
+    KITE_COLS_LDS(512)
+    KITE_COLS_LDS(640)
+    KITE_COLS_LDS(768)
+#undef KITE_COLS_LDS
+    default: return 0;
+    }
+}
+
+PassKernel fast_rows(int C) {
+    switch (C) {
+    case 64: return (PassKernel)k_rows_c<64>;
+    case 128: return (PassKernel)k_rows_c<128>;
+    case 256: return (PassKernel)k_rows_c<256>;
+    case 512: return (PassKernel)k_rows_c<512>;
+    case 1024: return (PassKernel)k_rows_c<1024>;
+    case 2048: return (PassKernel)k_rows_c<2048>;
+    default: return nullptr;
+    }
+}
+
+PassKernel fast_cols_inv(int R) {
+    switch (R) {
+    case 8: return (PassKernel)k_cols_inv_r<8>;
+    case 12: return (PassKernel)k_cols_inv_r<12>;
+    case 16: return (PassKernel)k_cols_inv_r<16>;
+    case 20: return (PassKernel)k_cols_inv_r<20>;
+    case 24: return (PassKernel)k_cols_inv_r<24>;
+    case 28: return (PassKernel)k_cols_inv_r<28>;
+    case 32: return (PassKernel)k_cols_inv_r<32>;
+    case 40: return (PassKernel)k_cols_inv_r<40>;
+    case 48: return (PassKernel)k_cols_inv_r<48>;
+    case 56: return (PassKernel)k_cols_inv_r<56>;
+    case 64: return (PassKernel)k_cols_inv_r<64>;
+    case 80: return (PassKernel)k_cols_inv_r<80>;
+    case 96: return (PassKernel)k_cols_inv_r<96>;
+    case 112: return (PassKernel)k_cols_inv_r<112>;
+    case 128: return (PassKernel)k_cols_inv_r<128>;
+    case 160: return (PassKernel)k_cols_inv_r<160>;
+    case 192: return (PassKernel)k_cols_inv_r<192>;
+    case 224: return (PassKernel)k_cols_inv_r<224>;
+    case 256: return (PassKernel)k_cols_inv_r<256>;
+    case 320: return (PassKernel)k_cols_inv_r<320>;
+    case 384: return (PassKernel)k_cols_inv_r<384>;
+    case 448: return (PassKernel)k_cols_inv_r<448>;
+    case 512: return (PassKernel)k_cols_inv_r<512>;
+    case 640: return (PassKernel)k_cols_inv_r<640>;
+    case 768: return (PassKernel)k_cols_inv_r<768>;
+    default: return nullptr;
+    }
+}
+
+PassKernel fast_cols_fwd(int R) {
+    switch (R) {
+    case 8: return (PassKernel)k_cols_fwd_r<8>;
+    case 12: return (PassKernel)k_cols_fwd_r<12>;
+    case 16: return (PassKernel)k_cols_fwd_r<16>;
+    case 20: return (PassKernel)k_cols_fwd_r<20>;
+    case 24: return (PassKernel)k_cols_fwd_r<24>;
+    case 28: return (PassKernel)k_cols_fwd_r<28>;
+    case 32: return (PassKernel)k_cols_fwd_r<32>;
+    case 40: return (PassKernel)k_cols_fwd_r<40>;
+    case 48: return (PassKernel)k_cols_fwd_r<48>;
+    case 56: return (PassKernel)k_cols_fwd_r<56>;
+    case 64: return (PassKernel)k_cols_fwd_r<64>;
+    case 80: return (PassKernel)k_cols_fwd_r<80>;
+    case 96: return (PassKernel)k_cols_fwd_r<96>;
+    case 112: return (PassKernel)k_cols_fwd_r<112>;
+    case 128: return (PassKernel)k_cols_fwd_r<128>;
+    case 160: return (PassKernel)k_cols_fwd_r<160>;
+    case 192: return (PassKernel)k_cols_fwd_r<192>;
+    case 224: return (PassKernel)k_cols_fwd_r<224>;
+    case 256: return (PassKernel)k_cols_fwd_r<256>;
+    case 320: return (PassKernel)k_cols_fwd_r<320>;
+    case 384: return (PassKernel)k_cols_fwd_r<384>;
+    case 448: return (PassKernel)k_cols_fwd_r<448>;
+    case 512: return (PassKernel)k_cols_fwd_r<512>;
+    case 640: return (PassKernel)k_cols_fwd_r<640>;
+    case 768: return (PassKernel)k_cols_fwd_r<768>;
+    default: return nullptr;
+    }
+}
+
+void all_kernels(PassKernel *out, int *count, int cap) {
+    int n = 0;
+    PassKernel g[3] = {generic_rows(), generic_cols_inv(), generic_cols_fwd()};
+    for (int i = 0; i < 3 && n < cap; i++) out[n++] = g[i];
+    const int Cs[] = {64, 128, 256, 512, 1024, 2048};
+    const int Rs[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96, 112, 128, 160, 192, 224, 256, 320, 384, 448, 512, 640, 768};
+    for (int c : Cs)
+        if (n < cap) out[n++] = fast_rows(c);
+    for (int r : Rs) {
+        if (n < cap) out[n++] = fast_cols_inv(r);
+        if (n < cap) out[n++] = fast_cols_fwd(r);
+    }
+    *count = n;
+}
+
+}  // namespace kite

@@ -67,6 +67,7 @@ bool fft_len_ok(uint32_t L);
 int engine_init(kite_ctx *ctx);
 void engine_free(kite_ctx *ctx);
 bool engine_shape_ok(uint32_t L, uint32_t top_digits);
+bool engine_shape_fast(uint32_t L);  // a shape with compile-time kernels exists
 void *engine_num_alloc(kite_ctx *ctx);
 void engine_num_free(kite_ctx *ctx, void *p);
 void engine_copy(kite_ctx *ctx, void *dst, const void *src);

@@ -239,7 +239,8 @@ extern "C" uint32_t kite_choose_fft_len(uint64_t k, uint64_t n) {
     double bc = 1e300;
     for (uint32_t L = lo; L <= lo + lo / 10 * 3; L += 64) {
         if (!usable(L)) continue;
-        double c = all ? tuned_us(L) : cost_model(L);
+        // untuned: the radix cost model, with lengths that only the generic kernels run priced higher
+        double c = all ? tuned_us(L) : cost_model(L) * (engine_shape_fast(L) ? 1.0 : 1.6);
         if (c < bc) {
             bc = c;
             best = L;

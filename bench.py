@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3000)
     ap.add_argument("--cpu-secs", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run a device copy of this many MiB (HBM-counter calibration in profiles)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (independent tests shard "
@@ -92,6 +94,15 @@ def main():
         sys.exit(2)
 
     from ahnum_amd import kite, workloads
+
+    if a.copy_calib_mb:
+        import torch
+        src = torch.empty(a.copy_calib_mb << 17, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     tests = workloads.single_large(a.batch)
 
