@@ -209,10 +209,20 @@ static ShapeEntry *get_shape(const ShapeChoice &ch) {
     while ((1ull << (2 * sh)) < H) sh++;
     h.sshift = sh;
     const uint64_t S = 1ull << sh, nhi = (H + S - 1) / S;
-    double2 *twR, *twC, *tlo, *thi, *twH;
+    // exact tables in the order the compile-time kernels read them (coalesced, shared by every
+    // test of this shape)
+    const std::vector<double2> wH = roots_table(H, H, 1);
+    std::vector<double2> trows(H), tcols(H);
+    for (uint64_t kr = 0; kr < ch.R; kr++)
+        for (uint64_t kc = 0; kc < ch.C; kc++) trows[kr * ch.C + kc] = wH[(kr + (uint64_t)ch.R * kc) % H];
+    for (uint64_t g = 0; g < ch.C / ch.G; g++)
+        for (uint64_t kr = 0; kr < ch.R; kr++)
+            for (uint64_t l = 0; l < ch.G; l++)
+                tcols[(g * ch.R + kr) * ch.G + l] = wH[((g * ch.G + l) * kr) % H];
+    double2 *twR, *twC, *tlo, *thi, *twRows, *twCols;
     if (upload(roots_table(ch.R, ch.R, 1), &twR, e->bufs) || upload(roots_table(ch.C, ch.C, 1), &twC, e->bufs) ||
         upload(roots_table(H, S, 1), &tlo, e->bufs) || upload(roots_table(H, nhi, S), &thi, e->bufs) ||
-        upload(roots_table(H, H, 1), &twH, e->bufs)) {
+        upload(trows, &twRows, e->bufs) || upload(tcols, &twCols, e->bufs)) {
         delete e;
         return nullptr;
     }
@@ -220,7 +230,8 @@ static ShapeEntry *get_shape(const ShapeChoice &ch) {
     h.twC = twC;
     h.tlo = tlo;
     h.thi = thi;
-    h.twH = twH;
+    h.twRows = twRows;
+    h.twCols = twCols;
     std::vector<ShapeDev> one(1, h);
     if (upload(one, &e->dev, e->bufs)) {
         delete e;

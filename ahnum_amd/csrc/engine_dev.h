@@ -67,7 +67,8 @@ struct ShapeDev {  // one (R, C, G) shape: H = R*C complex, NG = C/G column grou
     const double2 *twC;    // w_C^t, t < C
     const double2 *tlo;    // w_H^t, t < 2^sshift
     const double2 *thi;    // w_H^(t 2^sshift)
-    const double2 *twH;    // w_H^t, t < H (full table, used by the compile-time kernels)
+    const double2 *twRows; // [k_r][k_c] = w_H^(k_r + R k_c): split twiddles in row order
+    const double2 *twCols; // [g][k_r][l] = w_H^((g G + l) k_r): four-step twiddles in group order
 };
 
 struct SlotDev {  // one number being operated on (one test in a batch)

@@ -666,7 +666,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
     for (int it = 0; it < ITEMS; it++) {
         const int w = it * kNT + (int)threadIdx.x;
         const int i = w / C, kc = w - i * C, p = jb.y * PPW + i;
-        tw_[it] = (split && (NIT % kNT == 0 || w < NIT) && p < npairs) ? sh.twH[(p == 0 ? 0 : p) + R * kc]
+        tw_[it] = (split && (NIT % kNT == 0 || w < NIT) && p < npairs) ? sh.twRows[(size_t)p * C + kc]
                                                                           : make_double2(1, 0);
     }
     __syncthreads();
@@ -706,7 +706,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
             const int rh = row_of(0, 1);
             pb = C - 1 - kc;
             if (rh >= 0 && pb >= kc) {
-                const double2 t = sh.twH[rh + R * kc];
+                const double2 t = sh.twRows[(size_t)rh * C + kc];
                 double2 a = A[C + kc], b = A[C + pb];
                 if (mul) split_mul(a, b, t, s.yfreq[(size_t)rh * C + kc], s.yfreq[(size_t)rh * C + pb]);
                 else split_sq(a, b, t);
@@ -747,9 +747,10 @@ __device__ __forceinline__ void cols_inv_fast(const SlotDev *slots, const SlotDe
     int64_t *SC = reinterpret_cast<int64_t *>(T + R);
     int32_t *FD = reinterpret_cast<int32_t *>(SC + NSC);
     for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+    const double2 *tg = sh.twCols + (size_t)g * NE;
     for (int i = threadIdx.x; i < NE; i += kNT) {
         const int kr = i / G, l = i - kr * G;
-        A[i] = cmulc(s.x[(size_t)kr * C + c0 + l], sh.twH[(c0 + l) * kr]);
+        A[i] = cmulc(s.x[(size_t)kr * C + c0 + l], tg[i]);
     }
     __syncthreads();
     fft_ip<R, 1, 0, G, G, 1, true, true>(A, T);
@@ -851,8 +852,7 @@ __device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDe
     for (int it = 0; it < ITE; it++) {
         const int i = it * kNT + (int)threadIdx.x;
         if (NE % kNT != 0 && i >= NE) continue;
-        const int kr = i / G, l = i - kr * G;
-        tws[it] = sh.twH[(c0 + l) * kr];
+        tws[it] = sh.twCols[(size_t)g * NE + i];
     }
     const int32_t *in = s.dig + (size_t)g * R * 2 * G;
     const uint8_t *code = s.dcode + (size_t)g * R * G;
