@@ -391,7 +391,8 @@ int engine_init(kite_ctx *ctx) {
         E->ncoljobs = sh.NG;
         const int sub = sh.G < kSub ? sh.G : kSub;
         if (fr)  // single buffer of 2*ppw rows, then the row twiddles
-            E->lds_rows = (size_t)16 * (2 * ppw * sh.C + (sh.C <= (int)kTwLdsMaxC ? sh.C : 0));
+            E->lds_rows = (size_t)16 * (2 * ppw * sh.C + ((2 * ppw * sh.C) >> kRowPad) +
+                                         (sh.C <= (int)kTwLdsMaxC ? sh.C : 0));
         else
             E->lds_rows = (size_t)16 * (2 * 2 * sh.C);
         if (fi)

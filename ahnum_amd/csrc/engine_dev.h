@@ -16,6 +16,7 @@ constexpr int kMaxLds = 160 * 1024; // gfx950 LDS per work-group
 constexpr int kMaxTop = 64;         // top-region digits handled by the device fold
 constexpr int kSub = 8;             // elements (16 digits) per carry sub-chunk
 constexpr int kTwLdsMaxC = 1024;    // row twiddles are staged in LDS up to this row length
+constexpr int kRowPad = 3;          // row-kernel LDS: one padding element per 8
 
 // Radix plan of a length-n FFT: 8s first, then 4, 2, 3, 5, 7 (host make_plan uses the same order).
 __host__ __device__ constexpr int ct_nstages(int n) {

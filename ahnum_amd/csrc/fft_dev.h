@@ -586,7 +586,11 @@ __device__ __forceinline__ void cols_fwd_body(const SlotDev *slots, const SlotDe
 // in registers and moving its digits straight between HBM and registers, the full w_H table.
 // ================================================================================================
 
-template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV>
+// LDS index with PAD: one padding element after every 2^PAD (PAD = 0: none), which spreads the
+// strided writes of the early Stockham stages over distinct LDS bank groups.
+template <int PAD> __device__ __forceinline__ int padi(int i) { return PAD ? i + (i >> PAD) : i; }
+
+template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV, int PAD = 0>
 __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
     if constexpr (I < ct_nstages(N)) {
         constexpr int P = ct_radix(N, I), NB = N / P, TOTAL = NB * HOW;
@@ -600,7 +604,7 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
                 const int seq = SEQFAST ? w % HOW : w / NB, j = SEQFAST ? w / HOW : w % NB;
                 const int jm = j % NS;
 #pragma unroll
-                for (int r = 0; r < P; r++) v[it][r] = a[(j + r * NB) * IS + seq * SS];
+                for (int r = 0; r < P; r++) v[it][r] = a[padi<PAD>((j + r * NB) * IS + seq * SS)];
                 if (NS > 1 && jm) {
 #pragma unroll
                     for (int r = 1; r < P; r++) {
@@ -618,11 +622,11 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
             const int w = it * kNT + (int)threadIdx.x;
             if (TOTAL % kNT == 0 || w < TOTAL) {
 #pragma unroll
-                for (int r = 0; r < P; r++) a[ob[it] + r * NS * IS] = v[it][r];
+                for (int r = 0; r < P; r++) a[padi<PAD>(ob[it] + r * NS * IS)] = v[it][r];
             }
         }
         __syncthreads();
-        fft_ip<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV>(a, tw);
+        fft_ip<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV, PAD>(a, tw);
     }
 }
 
@@ -637,11 +641,12 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
     const ShapeDev &sh = *s.shp;
     constexpr int PPW = ppw_for_C(C), NROW = 2 * PPW, NE = NROW * C;
     constexpr bool TWL = C <= kTwLdsMaxC;
+    constexpr int PD = kRowPad;  // row buffer index i lives at i + i/8
     const int R = sh.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
     double2 *A = lds;
     const double2 *twC = sh.twC;
     if (TWL) {
-        double2 *T = lds + NE;
+        double2 *T = lds + padi<PD>(NE);
         for (int i = threadIdx.x; i < C; i += kNT) T[i] = sh.twC[i];
         twC = T;
     }
@@ -655,7 +660,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
     for (int e = threadIdx.x; e < NE; e += kNT) {
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
-        A[e] = row >= 0 ? x[(size_t)row * C + c] : make_double2(0, 0);
+        A[padi<PD>(e)] = row >= 0 ? x[(size_t)row * C + c] : make_double2(0, 0);
     }
     // split items: one per (local pair, k_c); the twiddle of item kc of pair p > 0 (and of row 0 of
     // pair 0) is fetched before the FFT
@@ -670,12 +675,12 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
                                                                           : make_double2(1, 0);
     }
     __syncthreads();
-    fft_ip<C, 1, 0, NROW, 1, C, false, false>(A, twC);
+    fft_ip<C, 1, 0, NROW, 1, C, false, false, PD>(A, twC);
     if (mode == kModeRowFft) {  // operand preparation: store the row spectra
         for (int e = threadIdx.x; e < NE; e += kNT) {
             const int slot = e / C, c = e - slot * C;
             const int row = row_of(slot >> 1, slot & 1);
-            if (row >= 0) s.yfreq[(size_t)row * C + c] = A[e];
+            if (row >= 0) s.yfreq[(size_t)row * C + c] = A[padi<PD>(e)];
         }
         return;
     }
@@ -688,39 +693,40 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
         if (p >= npairs) continue;
         if (p > 0) {  // (p, kc) with (R-p, C-1-kc)
             const int pb = C - 1 - kc;
-            double2 a = A[2 * i * C + kc], b = A[(2 * i + 1) * C + pb];
+            const int ea = padi<PD>(2 * i * C + kc), eb = padi<PD>((2 * i + 1) * C + pb);
+            double2 a = A[ea], b = A[eb];
             if (mul) split_mul(a, b, tw_[it], s.yfreq[(size_t)p * C + kc], s.yfreq[(size_t)(R - p) * C + pb]);
             else split_sq(a, b, tw_[it]);
-            A[2 * i * C + kc] = a;
-            A[(2 * i + 1) * C + pb] = b;
+            A[ea] = a;
+            A[eb] = b;
         } else {
             // row 0: kc with (C-kc) mod C; row R/2 (if present): kc with C-1-kc
             int pb = (C - kc) % C;
             if (pb >= kc) {
-                double2 a = A[kc], b = A[pb];
+                double2 a = A[padi<PD>(kc)], b = A[padi<PD>(pb)];
                 if (mul) split_mul(a, b, tw_[it], s.yfreq[kc], s.yfreq[pb]);
                 else split_sq(a, b, tw_[it]);
-                A[kc] = a;
-                if (pb != kc) A[pb] = b;
+                A[padi<PD>(kc)] = a;
+                if (pb != kc) A[padi<PD>(pb)] = b;
             }
             const int rh = row_of(0, 1);
             pb = C - 1 - kc;
             if (rh >= 0 && pb >= kc) {
                 const double2 t = sh.twRows[(size_t)rh * C + kc];
-                double2 a = A[C + kc], b = A[C + pb];
+                double2 a = A[padi<PD>(C + kc)], b = A[padi<PD>(C + pb)];
                 if (mul) split_mul(a, b, t, s.yfreq[(size_t)rh * C + kc], s.yfreq[(size_t)rh * C + pb]);
                 else split_sq(a, b, t);
-                A[C + kc] = a;
-                if (pb != kc) A[C + pb] = b;
+                A[padi<PD>(C + kc)] = a;
+                if (pb != kc) A[padi<PD>(C + pb)] = b;
             }
         }
     }
     __syncthreads();
-    fft_ip<C, 1, 0, NROW, 1, C, false, true>(A, twC);
+    fft_ip<C, 1, 0, NROW, 1, C, false, true, PD>(A, twC);
     for (int e = threadIdx.x; e < NE; e += kNT) {
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
-        if (row >= 0) s.x[(size_t)row * C + c] = A[e];
+        if (row >= 0) s.x[(size_t)row * C + c] = A[padi<PD>(e)];
     }
 }
 
