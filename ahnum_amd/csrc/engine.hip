@@ -255,6 +255,7 @@ struct Engine {
     int nrowjobs = 0, ncoljobs = 0;
     size_t lds_rows = 0, lds_cols = 0;
     PassKernel krows = nullptr, kcinv = nullptr, kcfwd = nullptr;
+    std::string name[3];  // kernel names of the three passes, for timing reports
     uint32_t H = 0;
 };
 
@@ -387,6 +388,9 @@ int engine_init(kite_ctx *ctx) {
         E->krows = fr ? fr : generic_rows();
         E->kcinv = fi ? fi : generic_cols_inv();
         E->kcfwd = ff ? ff : generic_cols_fwd();
+        E->name[0] = fr ? "k_rows_c<" + std::to_string(sh.C) + ">" : std::string("k_rows_g");
+        E->name[1] = fi ? "k_cols_inv_r<" + std::to_string(sh.R) + ">" : std::string("k_cols_inv_g");
+        E->name[2] = ff ? "k_cols_fwd_r<" + std::to_string(sh.R) + ">" : std::string("k_cols_fwd_g");
         E->nrowjobs = (npairs + ppw - 1) / ppw;
         E->ncoljobs = sh.NG;
         const int sub = sh.G < kSub ? sh.G : kSub;
@@ -550,14 +554,25 @@ struct kite_batch {
         kite::PassKernel k1, k2;  // rows: k1; columns: k1 = inverse, k2 = forward
         int off, count;
         size_t lds;
+        double sumL;        // sum of the FFT lengths of the group's tests
+        int stat[2];        // index into kstats of k1, k2
     };
     std::vector<Group> grows, gcols;
     unsigned long long *d_maxerr = nullptr;
     bool dirty = true;
-    bool timing = false;
+    int timing = 0;  // 0: off; N: every N-th step of a square call is timed
+    // timing: one event pair per launch of the last timed call, and what each pair measured
     std::vector<hipEvent_t> ev;
+    std::vector<int> ev_stat;
     double ms[3] = {0, 0, 0};
     uint64_t launches[3] = {0, 0, 0};
+    struct KStat {
+        std::string name;
+        int pass;
+        double ms = 0, bytes = 0;  // summed over launches: kernel time, algorithmic bytes
+        uint64_t launches = 0;
+    };
+    std::vector<KStat> kstats;
 };
 
 using namespace kite;
@@ -596,7 +611,7 @@ static int batch_refresh(kite_batch *b) {
             if (done[s0]) continue;
             Engine *E0 = b->slots[s0].ctx->eng;
             const PassKernel key = isrows ? E0->krows : E0->kcinv;
-            kite_batch::Group g{key, isrows ? nullptr : E0->kcfwd, (int)jobs.size(), 0, 0};
+            kite_batch::Group g{key, isrows ? nullptr : E0->kcfwd, (int)jobs.size(), 0, 0, 0.0, {-1, -1}};
             for (int s : order) {
                 Engine *E = b->slots[s].ctx->eng;
                 if (done[s] || (isrows ? E->krows : E->kcinv) != key) continue;
@@ -604,8 +619,24 @@ static int batch_refresh(kite_batch *b) {
                 const int nj = isrows ? E->nrowjobs : E->ncoljobs;
                 for (int u = 0; u < nj; u++) jobs.push_back(make_int2(s, u));
                 g.lds = std::max(g.lds, isrows ? E->lds_rows : E->lds_cols);
+                g.sumL += 2.0 * E->H;
             }
             g.count = (int)jobs.size() - g.off;
+            for (int i = 0; i < (isrows ? 1 : 2); i++) {
+                const int pass = isrows ? 0 : 1 + i;
+                const std::string &nm = E0->name[pass];
+                int at = -1;
+                for (size_t q = 0; q < b->kstats.size(); q++)
+                    if (b->kstats[q].name == nm && b->kstats[q].pass == pass) at = (int)q;
+                if (at < 0) {
+                    kite_batch::KStat ks;
+                    ks.name = nm;
+                    ks.pass = pass;
+                    b->kstats.push_back(ks);
+                    at = (int)b->kstats.size() - 1;
+                }
+                g.stat[i] = at;
+            }
             grp.push_back(g);
         }
     };
@@ -651,14 +682,28 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
     return 0;
 }
 
-static void batch_launch(kite_batch *b, int pass, int mode, int step) {
+// One launch per group of the pass; with `timed`, each launch is bracketed by an event pair.
+static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed = false) {
     SlotDev none{};
     const auto &groups = pass == 0 ? b->grows : b->gcols;
     const int2 *jobs = pass == 0 ? b->d_rows : b->d_cols;
     for (const auto &g : groups) {
         if (!g.count) continue;
         const PassKernel fn = pass == 2 ? g.k2 : g.k1;
-        launch_pass(fn, g.count, g.lds, b->stream, b->d_slots, none, jobs + g.off, mode, step);
+        if (timed) {
+            const size_t e = 2 * b->ev_stat.size();
+            if (b->ev.size() < e + 2) {
+                b->ev.resize(e + 2);
+                hipEventCreate(&b->ev[e]);
+                hipEventCreate(&b->ev[e + 1]);
+            }
+            b->ev_stat.push_back(g.stat[pass == 2 ? 1 : 0]);
+            hipEventRecord(b->ev[e], b->stream);
+            launch_pass(fn, g.count, g.lds, b->stream, b->d_slots, none, jobs + g.off, mode, step);
+            hipEventRecord(b->ev[e + 1], b->stream);
+        } else {
+            launch_pass(fn, g.count, g.lds, b->stream, b->d_slots, none, jobs + g.off, mode, step);
+        }
     }
 }
 
@@ -741,38 +786,45 @@ extern "C" kite_num *kite_batch_num(kite_batch *b, int slot, uint32_t i) {
 
 extern "C" int kite_batch_sync(kite_batch *b) { return hipStreamSynchronize(b->stream) == hipSuccess ? 0 : -1; }
 
-static void timed_launch(kite_batch *b, int pass, int mode, int step, size_t evi) {
-    if (b->timing) hipEventRecord(b->ev[evi], b->stream);
-    batch_launch(b, pass, mode, step);
-    if (b->timing) hipEventRecord(b->ev[evi + 1], b->stream);
-}
-
 extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint32_t *count) {
     uint32_t maxc = 0;
     for (size_t s = 0; s < b->slots.size(); s++)
         if (b->slots[s].ctx && which[s] >= 0) maxc = std::max(maxc, count[s]);
     if (!maxc) return 0;
     if (batch_upload(b, which, nullptr, count)) return -1;
-    if (b->timing && b->ev.size() < 6 * (size_t)maxc) {
-        size_t old = b->ev.size();
-        b->ev.resize(6 * (size_t)maxc);
-        for (size_t i = old; i < b->ev.size(); i++) hipEventCreate(&b->ev[i]);
-    }
+    b->ev_stat.clear();
     for (uint32_t st = 0; st < maxc; st++) {
-        timed_launch(b, 0, kModeSquare, (int)st, 6 * st);
-        timed_launch(b, 1, kModeSquare, (int)st, 6 * st + 2);
-        timed_launch(b, 2, kModeSquare, (int)st, 6 * st + 4);
+        const bool timed = b->timing && st % (uint32_t)b->timing == 0;
+        batch_launch(b, 0, kModeSquare, (int)st, timed);
+        batch_launch(b, 1, kModeSquare, (int)st, timed);
+        batch_launch(b, 2, kModeSquare, (int)st, timed);
     }
     if (b->timing) {
+        // per-pass and per-kernel totals of this call; algorithmic bytes per test and launch are
+        // 16 L for the row pass (residue read + written) and 8 L for each column pass
         hipStreamSynchronize(b->stream);
-        for (int p = 0; p < 3; p++) b->ms[p] = 0;
-        for (uint32_t st = 0; st < maxc; st++)
-            for (int p = 0; p < 3; p++) {
-                float ms = 0;
-                hipEventElapsedTime(&ms, b->ev[6 * st + 2 * p], b->ev[6 * st + 2 * p + 1]);
-                b->ms[p] += ms;
+        for (int p = 0; p < 3; p++) {
+            b->ms[p] = 0;
+            b->launches[p] = (maxc + b->timing - 1) / b->timing;
+        }
+        for (auto &ks : b->kstats) ks = kite_batch::KStat{ks.name, ks.pass};
+        std::vector<double> bytes(b->kstats.size(), 0.0);
+        for (const auto &g : b->grows)
+            if (g.count) bytes[g.stat[0]] += 16.0 * g.sumL;
+        for (const auto &g : b->gcols)
+            if (g.count) {
+                bytes[g.stat[0]] += 8.0 * g.sumL;
+                bytes[g.stat[1]] += 8.0 * g.sumL;
             }
-        for (int p = 0; p < 3; p++) b->launches[p] = maxc;
+        for (size_t i = 0; i < b->ev_stat.size(); i++) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]);
+            kite_batch::KStat &ks = b->kstats[b->ev_stat[i]];
+            ks.ms += ms;
+            ks.launches++;
+            ks.bytes += bytes[b->ev_stat[i]];
+            b->ms[ks.pass] += ms;
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -816,8 +868,8 @@ extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
     return 0;
 }
 
-extern "C" int kite_batch_set_timing(kite_batch *b, int on) {
-    b->timing = on != 0;
+extern "C" int kite_batch_set_timing(kite_batch *b, int every) {
+    b->timing = every > 0 ? every : 0;
     return 0;
 }
 
@@ -827,4 +879,19 @@ extern "C" int kite_batch_pass_times(kite_batch *b, double *out3, uint64_t *laun
         if (launches3) launches3[p] = b->launches[p];
     }
     return 0;
+}
+
+extern "C" int kite_batch_kernel_times(kite_batch *b, int i, char *name, size_t name_len, double *ms,
+                                       uint64_t *launches, double *bytes) {
+    if (i < 0) return (int)b->kstats.size();
+    if ((size_t)i >= b->kstats.size()) return -1;
+    const kite_batch::KStat &ks = b->kstats[i];
+    if (name && name_len) {
+        strncpy(name, ks.name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (ms) *ms = ks.ms;
+    if (launches) *launches = ks.launches;
+    if (bytes) *bytes = ks.bytes;
+    return (int)b->kstats.size();
 }

@@ -327,6 +327,9 @@ SIGNATURES.update({
     "kite_batch_sync": (ctypes.c_int, [c_p]),
     "kite_batch_set_timing": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_pass_times": (ctypes.c_int, [c_p, ctypes.POINTER(c_dbl), ctypes.POINTER(c_u64)]),
+    "kite_batch_kernel_times": (ctypes.c_int, [c_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                                               ctypes.POINTER(c_dbl), ctypes.POINTER(c_u64),
+                                               ctypes.POINTER(c_dbl)]),
     "kite_batch_prp": (ctypes.c_int, [ctypes.POINTER(BatchTest), ctypes.c_size_t, ctypes.POINTER(BatchPrpOpts),
                                       ctypes.POINTER(PrpResult), ctypes.POINTER(ctypes.c_int)]),
 })
@@ -404,14 +407,28 @@ class Batch:
     def sync(self):
         self.lib.kite_batch_sync(self.p)
 
-    def timing(self, on):
-        self.lib.kite_batch_set_timing(self.p, int(on))
+    def timing(self, every):
+        """Time each launch of every `every`-th step of later square calls (0/False: off)."""
+        self.lib.kite_batch_set_timing(self.p, int(every))
 
     def pass_times(self):
         ms = (c_dbl * 3)()
         n = (c_u64 * 3)()
         self.lib.kite_batch_pass_times(self.p, ms, n)
         return list(ms), list(n)
+
+    def kernel_times(self):
+        """[(name, total_ms, launches, algorithmic_bytes)] of the last timed square call."""
+        out = []
+        cnt = self.lib.kite_batch_kernel_times(self.p, -1, None, 0, None, None, None)
+        for i in range(cnt):
+            name = ctypes.create_string_buffer(64)
+            ms, ln, by = c_dbl(), c_u64(), c_dbl()
+            self.lib.kite_batch_kernel_times(self.p, i, name, 64, ctypes.byref(ms), ctypes.byref(ln),
+                                             ctypes.byref(by))
+            if ln.value:
+                out.append((name.value.decode(), ms.value, ln.value, by.value))
+        return out
 
     def close(self):
         if self.p:

@@ -9,10 +9,12 @@ launch per pass. The same number is also timed alone through the kite.h path (th
 one-test-at-a-time GPU path). tests/hour follows the reference's throughput formula
 (tools/kite.c:262): sum over tests of squarings/s * 3600 / n.
 
-Prints one JSON line. `roofline` is for the dominant pass kernel, timed with HIP events on the
-batch stream over the timed region; algorithmic bytes per launch are 16 L per test for the row
-pass (read + write of the 8L-byte residue) and 8 L per test for each column pass (32 L per
-squaring in total). `cpu_baseline` is the oracle's restatement of the reference's portable
+Prints one JSON line. `roofline` is for the dominant pass kernel (the most device time), timed
+with HIP events on the batch stream around each launch of every 10th step of the timed region;
+algorithmic bytes per launch are 16 L per test for the row pass (read + write of the 8L-byte
+residue) and 8 L per test for each column pass (32 L per squaring in total). `traffic` is the
+kernel's HBM bytes per launch from profiles/hbm_traffic_single_large_batch.txt (rocprofv3
+FETCH_SIZE and WRITE_SIZE runs of this benchmark, calibrated by a copy of known size). `cpu_baseline` is the oracle's restatement of the reference's portable
 two-pass backend (--backend c2p) timed on the host cores, one test per core.
 """
 import argparse
@@ -64,16 +66,21 @@ def cpu_baseline(tests, secs):
     }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_large_batch.txt")
+
+
 def load_traffic(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this command, if any."""
-    p = os.path.join(ROOT, "profiles", f"{kernel}_pmc_single_large_batch.json")
-    if not os.path.exists(p):
-        return None
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    of this benchmark (lines "traffic <kernel> <bytes per launch>"), or None."""
     try:
-        with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic" and w[1] == kernel:
+                    return float(w[2])
     except (OSError, ValueError):
-        return None
+        pass
+    return None
 
 
 def main():
@@ -85,8 +92,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3000)
     ap.add_argument("--cpu-secs", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-every", type=int, default=10,
+                    help="bracket the launches of every N-th timed step with HIP events")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
-                    help="also run a device copy of this many MiB (HBM-counter calibration in profiles)")
+                    help="also run a fill, a copy and an add over this many MiB (calibrates the HBM "
+                         "byte counters of a rocprofv3 --pmc run of this command)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (independent tests shard "
@@ -99,8 +109,9 @@ def main():
         import torch
         src = torch.empty(a.copy_calib_mb << 17, dtype=torch.float64, device="cuda:0").fill_(1.0)
         dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
+        for _ in range(3):
+            dst.copy_(src)  # reads and writes copy_calib_mb MiB
+            torch.add(src, 1.0, out=dst)  # elementwise kernel: same bytes
         torch.cuda.synchronize()
         del src, dst
 
@@ -131,25 +142,26 @@ def main():
         Ls.append(b.fft_len(s))
     b.square(a.warmup)
     b.maxerr(reset=True)
-    b.timing(True)
+    b.timing(a.time_every)
     b.sync()
     t0 = time.perf_counter()
     b.square(a.steps)
     b.sync()
     wall = time.perf_counter() - t0
     ms, launches = b.pass_times()
+    kstats = b.kernel_times()
     errs = b.maxerr()
     b.close()
 
     rate = a.steps / wall  # squarings/s per test
     tph = sum(rate * 3600.0 / n for _, n in tests)
-    names = ["k_rows", "k_cols_inv", "k_cols_fwd"]
-    per_test_bytes = [16, 8, 8]
-    dom = max(range(3), key=lambda p: ms[p])
-    avg_s = ms[dom] / 1e3 / launches[dom]
-    alg_bytes = sum(per_test_bytes[dom] * L for L in Ls)
+    names = ["rows", "cols_inv", "cols_fwd"]
+    # dominant kernel: the most device time over the timed region (HIP events on the batch stream)
+    kname, kms, klaunch, kbytes = max(kstats, key=lambda t: t[1])
+    avg_s = kms / 1e3 / klaunch
+    alg_bytes = kbytes / klaunch
     achieved = alg_bytes / avg_s / 1e9
-    traffic = load_traffic(names[dom])
+    traffic = load_traffic(kname)
 
     out = {
         "metric": "tests/hour at n~660K (Fermat base 3, FFT 49152, Gerbicz-checkable residues), batched",
@@ -172,14 +184,17 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": names[dom],
+            "kernel": kname,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(alg_bytes),
             "avg_launch_us": round(avg_s * 1e6, 2),
             "pass_ms": {names[p]: round(ms[p], 3) for p in range(3)},
+            "kernels": {n: {"avg_us": round(t / c * 1e3, 2), "launches": c,
+                            "gbs": round(by / (t / 1e3) / 1e9, 1)} for n, t, c, by in kstats},
         },
         "plain": {
             "us_per_squaring": round(plain_us, 3),

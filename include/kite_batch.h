@@ -40,10 +40,16 @@ int kite_batch_copy(kite_batch *b, const int32_t *dst, const int32_t *src);
 int kite_batch_maxerr(kite_batch *b, double *out, int reset);
 int kite_batch_sync(kite_batch *b);
 // Device time of each pass over the last kite_batch_square call timed with HIP events (ms):
-// out[0] rows, out[1] column inverse + carry, out[2] column forward; with timing enabled every
-// launch is bracketed by events on the batch stream.
-int kite_batch_set_timing(kite_batch *b, int on);
+// out[0] rows, out[1] column inverse + carry, out[2] column forward. With timing every N > 0,
+// each launch of every N-th step is bracketed by events on the batch stream (0 turns it off);
+// launches3 counts the timed launches.
+int kite_batch_set_timing(kite_batch *b, int every);
 int kite_batch_pass_times(kite_batch *b, double *out3, uint64_t *launches3);
+// Per-kernel totals over the same timed call: kernel i's name, summed device time (ms), number of
+// launches and summed algorithmic bytes (16 L per test for a row launch, 8 L per test for each
+// column launch). Returns the number of kernels (i < 0 only queries it), -1 for i out of range.
+int kite_batch_kernel_times(kite_batch *b, int i, char *name, size_t name_len, double *ms,
+                            uint64_t *launches, double *bytes);
 
 // ---- batched Fermat PRP tests ---------------------------------------------------------------
 
