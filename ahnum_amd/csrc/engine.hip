@@ -255,6 +255,7 @@ struct Engine {
     int nrowjobs = 0, ncoljobs = 0;
     size_t lds_rows = 0, lds_cols = 0;
     PassKernel krows = nullptr, kcinv = nullptr, kcfwd = nullptr;
+    PassKernel krows_sq = nullptr;  // square-mode rows with registers (64-thread groups), if any
     std::string name[3];  // kernel names of the three passes, for timing reports
     uint32_t H = 0;
 };
@@ -386,9 +387,12 @@ int engine_init(kite_ctx *ctx) {
                    ff = gct ? fast_cols_fwd(sh.R) : nullptr;
         const int ppw = fr ? ppw_for_C(sh.C) : 1;
         E->krows = fr ? fr : generic_rows();
+        E->krows_sq = (fr && ppw_for_C(sh.C) == 4) ? fast_rows_sq(sh.C) : nullptr;
         E->kcinv = fi ? fi : generic_cols_inv();
         E->kcfwd = ff ? ff : generic_cols_fwd();
-        E->name[0] = fr ? "k_rows_c<" + std::to_string(sh.C) + ">" : std::string("k_rows_g");
+        E->name[0] = E->krows_sq ? "k_rows_sq" + std::to_string(sh.C)
+                     : fr       ? "k_rows_c<" + std::to_string(sh.C) + ">"
+                                : std::string("k_rows_g");
         E->name[1] = fi ? "k_cols_inv_r<" + std::to_string(sh.R) + ">" : std::string("k_cols_inv_g");
         E->name[2] = ff ? "k_cols_fwd_r<" + std::to_string(sh.R) + ">" : std::string("k_cols_fwd_g");
         E->nrowjobs = (npairs + ppw - 1) / ppw;
@@ -438,16 +442,19 @@ void engine_copy(kite_ctx *ctx, void *dst, const void *src) {
 }
 
 static void launch_pass(PassKernel fn, int grid, size_t lds, hipStream_t st, const SlotDev *slots,
-                        const SlotDev &one, const int2 *jobs, int mode, int step) {
+                        const SlotDev &one, const int2 *jobs, int mode, int step, int threads = kNT) {
     void *args[] = {(void *)&slots, (void *)&one, (void *)&jobs, (void *)&mode, (void *)&step};
-    hipLaunchKernel(fn, dim3(grid), dim3(kNT), args, lds, st);
+    hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st);
 }
 
 static void launch_rows(Engine *E, double2 *x, const double2 *ysrc, int mode) {
     SlotDev s = E->slot;
     s.x = x;
     s.y_src = ysrc;
-    launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, nullptr, s, nullptr, mode, 0);
+    if (mode == kModeSquare && E->krows_sq)
+        launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, nullptr, s, nullptr, mode, 0, kRowsSqThreads);
+    else
+        launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, nullptr, s, nullptr, mode, 0);
 }
 
 static void launch_cols(Engine *E, double2 *x, int mode) {
@@ -551,7 +558,7 @@ struct kite_batch {
     int nrows = 0, ncols = 0;
     size_t jobs_cap = 0;
     struct Group {
-        kite::PassKernel k1, k2;  // rows: k1; columns: k1 = inverse, k2 = forward
+        kite::PassKernel k1, k2;  // rows: k1 (k2: square-mode kernel, if any); columns: k1 = inverse, k2 = forward
         int off, count;
         size_t lds;
         double sumL;        // sum of the FFT lengths of the group's tests
@@ -611,7 +618,7 @@ static int batch_refresh(kite_batch *b) {
             if (done[s0]) continue;
             Engine *E0 = b->slots[s0].ctx->eng;
             const PassKernel key = isrows ? E0->krows : E0->kcinv;
-            kite_batch::Group g{key, isrows ? nullptr : E0->kcfwd, (int)jobs.size(), 0, 0, 0.0, {-1, -1}};
+            kite_batch::Group g{key, isrows ? E0->krows_sq : E0->kcfwd, (int)jobs.size(), 0, 0, 0.0, {-1, -1}};
             for (int s : order) {
                 Engine *E = b->slots[s].ctx->eng;
                 if (done[s] || (isrows ? E->krows : E->kcinv) != key) continue;
@@ -689,7 +696,10 @@ static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed
     const int2 *jobs = pass == 0 ? b->d_rows : b->d_cols;
     for (const auto &g : groups) {
         if (!g.count) continue;
-        const PassKernel fn = pass == 2 ? g.k2 : g.k1;
+        const bool rsq = pass == 0 && mode == kModeSquare && g.k2;
+        const PassKernel fn = (pass == 2 || rsq) ? g.k2 : g.k1;
+        const size_t lds = rsq ? 0 : g.lds;
+        const int threads = rsq ? kRowsSqThreads : kNT;
         if (timed) {
             const size_t e = 2 * b->ev_stat.size();
             if (b->ev.size() < e + 2) {
@@ -699,10 +709,10 @@ static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed
             }
             b->ev_stat.push_back(g.stat[pass == 2 ? 1 : 0]);
             hipEventRecord(b->ev[e], b->stream);
-            launch_pass(fn, g.count, g.lds, b->stream, b->d_slots, none, jobs + g.off, mode, step);
+            launch_pass(fn, g.count, lds, b->stream, b->d_slots, none, jobs + g.off, mode, step, threads);
             hipEventRecord(b->ev[e + 1], b->stream);
         } else {
-            launch_pass(fn, g.count, g.lds, b->stream, b->d_slots, none, jobs + g.off, mode, step);
+            launch_pass(fn, g.count, lds, b->stream, b->d_slots, none, jobs + g.off, mode, step, threads);
         }
     }
 }

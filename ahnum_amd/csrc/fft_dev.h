@@ -84,6 +84,64 @@ template <bool INV> struct Dft<8, INV> {
     }
 };
 
+// v * w16^E (w16 = e^{-+2 pi i/16}) for a compile-time E: exact for multiples of 4, two real
+// multiplications for the odd multiples of 2, a complex multiplication otherwise.
+template <int E, bool INV> __device__ __forceinline__ double2 mul_w16(double2 v) {
+    constexpr int e = E & 15;
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    if constexpr (e == 0) {
+        return v;
+    } else if constexpr (e == 4) {
+        return mul_mi<INV>(v);
+    } else if constexpr (e == 8) {
+        return make_double2(-v.x, -v.y);
+    } else if constexpr (e == 12) {
+        return mul_mi<!INV>(v);
+    } else if constexpr (e % 2 == 0) {
+        // (cos, sin) of 2 pi e/16 for e = 2, 6, 10, 14 are (+-h, +-h)
+        constexpr double c = (e == 2 || e == 14) ? h : -h, s = (e == 2 || e == 6) ? h : -h;
+        constexpr double sg = INV ? s : -s;  // multiplier c + i sg
+        if constexpr (c == sg) return make_double2(c * (v.x - v.y), c * (v.x + v.y));
+        else return make_double2(c * (v.x + v.y), c * (v.y - v.x));
+    } else {
+        constexpr int q = e % 4;  // 1: angle pi/8 off an axis, 3: 3 pi/8
+        constexpr double ca = q == 1 ? c1 : s1, sa = q == 1 ? s1 : c1;
+        // cos/sin of 2 pi e/16 from the quadrant of e
+        constexpr int quad = e / 4;
+        constexpr double c = quad == 0 ? ca : quad == 1 ? -sa : quad == 2 ? -ca : sa;
+        constexpr double s = quad == 0 ? sa : quad == 1 ? ca : quad == 2 ? -sa : -ca;
+        constexpr double sg = INV ? s : -s;
+        return make_double2(v.x * c - v.y * sg, v.x * sg + v.y * c);
+    }
+}
+
+// 16-point DFT as 4 x 4: n = 4a + b, k = k1 + 4 k2; y[b][k1] = DFT4_a(v[4a+b]) w16^{b k1}, then
+// v[k1 + 4 k2] = DFT4_b(y[b][k1]).
+template <bool INV> struct Dft<16, INV> {
+    template <int B> static __device__ __forceinline__ void inner(const double2 *v, double2 *y) {
+        double2 t[4] = {v[B], v[4 + B], v[8 + B], v[12 + B]};
+        Dft<4, INV>::run(t);
+        y[4 * B + 0] = t[0];
+        y[4 * B + 1] = mul_w16<B * 1, INV>(t[1]);
+        y[4 * B + 2] = mul_w16<B * 2, INV>(t[2]);
+        y[4 * B + 3] = mul_w16<B * 3, INV>(t[3]);
+    }
+    static __device__ __forceinline__ void run(double2 *v) {
+        double2 y[16];
+        inner<0>(v, y);
+        inner<1>(v, y);
+        inner<2>(v, y);
+        inner<3>(v, y);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            double2 t[4] = {y[k1], y[4 + k1], y[8 + k1], y[12 + k1]};
+            Dft<4, INV>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[k2];
+        }
+    }
+};
+
 template <int P> struct OddRoots;
 template <> struct OddRoots<3> {
     static __device__ __forceinline__ double c(int) { return -0.5; }
@@ -727,6 +785,127 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
         if (row >= 0) s.x[(size_t)row * C + c] = A[padi<PD>(e)];
+    }
+}
+
+// Square-mode rows for C = 256 held in registers. A work-group is two waves of four 16-lane groups;
+// each group takes one row: group i of wave 0 row k_r and group i of wave 1 row R - k_r of local
+// pair i (four pairs per work-group, the job granularity of the LDS row kernel), so the side of a
+// pair is uniform over a wave. Lane t loads elements 16a + t (a < 16),
+// so every load and store moves 256 contiguous bytes per group. Forward row FFT as 16 x 16: DFT16
+// over a in registers, twiddle w_256^{t k1}, a transpose through the group's LDS block (slot
+// k1*16 + (b ^ k1): conflict-free both ways), DFT16 over b. Lane t then holds k_c = t + 16 k2 of
+// row k_r and k_c = (15 - t) + 16 k2 of row R - k_r. For the real split each group copies its
+// row to LDS in natural order and every lane reads the partners of its own elements
+// (k_c <-> C - 1 - k_c in the partner row; within the row for the self-paired rows 0 and R/2)
+// and computes its own half of each pair. The inverse runs the forward steps backwards.
+constexpr int kRowsRegC = 256;
+
+// Loads and stores of device memory through global-address-space pointers: global (not flat)
+// instructions, which wait on the memory counter only.
+typedef double dv2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 gld(const double2 *p) {
+    const dv2_t v = *(const __attribute__((address_space(1))) dv2_t *)p;
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(double2 *p, double2 v) {
+    dv2_t w;
+    w.x = v.x;
+    w.y = v.y;
+    *(__attribute__((address_space(1))) dv2_t *)p = w;
+}
+
+// The half of the real-split square (split_sq) that belongs to `u`: u is side a (partner w is
+// b) or, with isb, side b; tw = w_H^k of side a's frequency k.
+__device__ __forceinline__ double2 split_sq_half(double2 u, double2 w, double2 tw, bool isb) {
+    const double2 a = isb ? w : u, b = isb ? u : w;
+    const double2 e = make_double2(a.x + b.x, a.y - b.y);
+    const double2 o = make_double2(a.y + b.y, b.x - a.x);
+    const double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2 * e.x * e.y);
+    const double2 o2 = make_double2(o.x * o.x - o.y * o.y, 2 * o.x * o.y);
+    const double2 X = cadd(e2, cmul(tw, o2));
+    const double2 Y = make_double2(2 * (e.x * o.x - e.y * o.y), 2 * (e.x * o.y + e.y * o.x));
+    return isb ? make_double2(X.x + Y.y, Y.x - X.y) : make_double2(X.x - Y.y, X.y + Y.x);
+}
+
+// w_H^{p + R k_c} = T0 w16^{k2} for side a's k_c = t + 16 k2 (H = 256 R, T0 = w_H^{p + R t}); the
+// side-b element k2 pairs with side a's k2' = 15 - k2 of the same t.
+template <int K2, bool SIDE_B>
+__device__ __forceinline__ void rows256_split_step(double2 *v, const double2 *Xp, int km, double2 T0) {
+    if constexpr (K2 < 16) {
+        const int kc = km + 16 * K2;
+        v[K2] = split_sq_half(v[K2], Xp[kRowsRegC - 1 - kc], mul_w16<SIDE_B ? 15 - K2 : K2, false>(T0), SIDE_B);
+        if constexpr (K2 % 4 == 3) __builtin_amdgcn_sched_barrier(0);  // bounds the hoisting of LDS reads
+        rows256_split_step<K2 + 1, SIDE_B>(v, Xp, km, T0);
+    }
+}
+
+__device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int step) {
+    constexpr int C = kRowsRegC;
+    __shared__ double2 xs[8 * 256];
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;
+    const ShapeDev &sh = *s.shp;
+    const int lane = (int)threadIdx.x, g = lane >> 4, t = lane & 15;
+    const int side = g >> 2, p = jb.y * 4 + (g & 3);
+    const int R = sh.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
+    const bool special = p == 0;  // rows 0 and R/2, each paired with itself
+    int row = -1;
+    if (p < npairs) row = side == 0 ? p : (special ? (R % 2 == 0 ? R / 2 : -1) : R - p);
+    const bool act = row >= 0;
+    const int km = (side == 1 && !special) ? 15 - t : t;  // k1 held after the transpose
+    double2 *X = xs + g * 256;
+    double2 *xr = s.x + (size_t)(act ? row : 0) * C + t;
+    const double2 *twC = sh.twC;
+    double2 v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = act ? gld(xr + 16 * i) : make_double2(0, 0);
+    // forward: DFT16 over a, twiddle w_256^{t k1}, transpose, DFT16 over b
+    Dft<16, false>::run(v);
+#pragma unroll
+    for (int j = 1; j < 16; j++) v[j] = cmul(v[j], gld(twC + t * j));
+#pragma unroll
+    for (int j = 0; j < 16; j++) X[j * 16 + (t ^ j)] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = X[km * 16 + (j ^ km)];
+    __syncthreads();
+    Dft<16, false>::run(v);  // v[k2] = U[row][km + 16 k2]
+    // real-split square against the partners read from LDS
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) X[km + 16 * k2] = v[k2];
+    __syncthreads();
+    if (act) {
+        if (!special) {
+            const double2 T0 = gld(sh.twRows + (size_t)p * C + t);
+            if (side == 0) rows256_split_step<0, false>(v, xs + (g ^ 4) * 256, km, T0);
+            else rows256_split_step<0, true>(v, xs + (g ^ 4) * 256, km, T0);
+        } else {
+            const double2 *tw = sh.twRows + (size_t)row * C;
+#pragma unroll
+            for (int k2 = 0; k2 < 16; k2++) {
+                const int kc = t + 16 * k2, pb = side == 0 ? (C - kc) & (C - 1) : C - 1 - kc;
+                const bool isb = kc > pb;
+                v[k2] = split_sq_half(v[k2], X[pb], gld(tw + (isb ? pb : kc)), isb);
+                if (k2 % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    __syncthreads();
+    // inverse: DFT16 over k2, twiddle w_256^{-k1 b}, transpose, DFT16 over k1
+    Dft<16, true>::run(v);
+#pragma unroll
+    for (int j = 1; j < 16; j++) v[j] = cmulc(v[j], gld(twC + km * j));
+#pragma unroll
+    for (int j = 0; j < 16; j++) X[j * 16 + (km ^ j)] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = X[t * 16 + (j ^ t)];
+    Dft<16, true>::run(v);
+    if (act) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) gst(xr + 16 * i, v[i]);
     }
 }
 

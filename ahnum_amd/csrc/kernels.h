@@ -16,6 +16,10 @@ PassKernel generic_cols_inv();
 PassKernel generic_cols_fwd();
 // nullptr when no compile-time kernel exists for that size
 PassKernel fast_rows(int C);
+// square-mode row kernel with the rows held in registers (128 threads per work-group, four row
+// pairs like the LDS row kernel, static LDS only); nullptr for row lengths without one
+PassKernel fast_rows_sq(int C);
+constexpr int kRowsSqThreads = 128;
 PassKernel fast_cols_inv(int R);
 PassKernel fast_cols_fwd(int R);
 size_t fast_cols_lds(int R);  // LDS bytes of the compile-time column kernels

@@ -10,6 +10,10 @@ template <int C>
 __global__ void __launch_bounds__(kNT) k_rows_c(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
     rows_fast<C>(slots, one, jobs, mode, step);
 }
+__global__ void __launch_bounds__(128, 3) k_rows_sq256(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    (void)mode;
+    rows256_sq(slots, one, jobs, step);
+}
 template <int R>
 __global__ void __launch_bounds__(kNT) k_cols_inv_r(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
     cols_inv_fast<R>(slots, one, jobs, mode, step);
@@ -63,6 +67,8 @@ PassKernel fast_rows(int C) {
     default: return nullptr;
     }
 }
+
+PassKernel fast_rows_sq(int C) { return C == kRowsRegC ? (PassKernel)k_rows_sq256 : nullptr; }
 
 PassKernel fast_cols_inv(int R) {
     switch (R) {
