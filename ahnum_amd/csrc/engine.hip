@@ -610,16 +610,31 @@ static int batch_refresh(kite_batch *b) {
     std::vector<int> order;
     for (size_t s = 0; s < cap; s++)
         if (b->slots[s].ctx) order.push_back((int)s);
+    auto stat_of = [&](const std::string &nm, int pass) {
+        for (size_t q = 0; q < b->kstats.size(); q++)
+            if (b->kstats[q].name == nm && b->kstats[q].pass == pass) return (int)q;
+        kite_batch::KStat ks;
+        ks.name = nm;
+        ks.pass = pass;
+        b->kstats.push_back(ks);
+        return (int)b->kstats.size() - 1;
+    };
     auto build = [&](bool isrows) {
         std::vector<int2> &jobs = isrows ? rows : cols;
         std::vector<kite_batch::Group> &grp = isrows ? b->grows : b->gcols;
+        std::vector<int> gR;  // column length of each group
         std::vector<bool> done(cap, false);
-        for (int s0 : order) {
+        std::vector<int> ord = order;
+        if (!isrows)  // column groups in ascending R, so that adjacent lengths sit side by side
+            std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+                return b->slots[x].ctx->eng->shape->ch.R < b->slots[y].ctx->eng->shape->ch.R;
+            });
+        for (int s0 : ord) {
             if (done[s0]) continue;
             Engine *E0 = b->slots[s0].ctx->eng;
             const PassKernel key = isrows ? E0->krows : E0->kcinv;
             kite_batch::Group g{key, isrows ? E0->krows_sq : E0->kcfwd, (int)jobs.size(), 0, 0, 0.0, {-1, -1}};
-            for (int s : order) {
+            for (int s : ord) {
                 Engine *E = b->slots[s].ctx->eng;
                 if (done[s] || (isrows ? E->krows : E->kcinv) != key) continue;
                 done[s] = true;
@@ -629,23 +644,38 @@ static int batch_refresh(kite_batch *b) {
                 g.sumL += 2.0 * E->H;
             }
             g.count = (int)jobs.size() - g.off;
-            for (int i = 0; i < (isrows ? 1 : 2); i++) {
-                const int pass = isrows ? 0 : 1 + i;
-                const std::string &nm = E0->name[pass];
-                int at = -1;
-                for (size_t q = 0; q < b->kstats.size(); q++)
-                    if (b->kstats[q].name == nm && b->kstats[q].pass == pass) at = (int)q;
-                if (at < 0) {
-                    kite_batch::KStat ks;
-                    ks.name = nm;
-                    ks.pass = pass;
-                    b->kstats.push_back(ks);
-                    at = (int)b->kstats.size() - 1;
-                }
-                g.stat[i] = at;
+            if (isrows) {
+                g.stat[0] = stat_of(E0->name[0], 0);
+            } else {
+                g.stat[0] = stat_of(E0->name[1], 1);
+                g.stat[1] = stat_of(E0->name[2], 2);
             }
             grp.push_back(g);
+            gR.push_back((int)E0->shape->ch.R);
         }
+        if (isrows) return;
+        // adjacent compile-time column lengths share one launch per pass
+        std::vector<kite_batch::Group> merged;
+        for (size_t i = 0; i < grp.size(); i++) {
+            const int r1 = gR[i];
+            if (i + 1 < grp.size() && grp[i].k1 == fast_cols_inv(r1) && fast_cols_next(r1) == gR[i + 1] &&
+                grp[i + 1].k1 == fast_cols_inv(gR[i + 1])) {
+                kite_batch::Group g = grp[i];
+                g.k1 = fast_cols_pair_inv(r1);
+                g.k2 = fast_cols_pair_fwd(r1);
+                g.count += grp[i + 1].count;
+                g.lds = std::max(g.lds, grp[i + 1].lds);
+                g.sumL += grp[i + 1].sumL;
+                const std::string rr = "<" + std::to_string(r1) + "," + std::to_string(gR[i + 1]) + ">";
+                g.stat[0] = stat_of("k_cols_inv_rr" + rr, 1);
+                g.stat[1] = stat_of("k_cols_fwd_rr" + rr, 2);
+                merged.push_back(g);
+                i++;
+            } else {
+                merged.push_back(grp[i]);
+            }
+        }
+        grp = merged;
     };
     build(true);
     build(false);

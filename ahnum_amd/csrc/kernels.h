@@ -23,6 +23,11 @@ constexpr int kRowsSqThreads = 128;
 PassKernel fast_cols_inv(int R);
 PassKernel fast_cols_fwd(int R);
 size_t fast_cols_lds(int R);  // LDS bytes of the compile-time column kernels
+// the column length after R with compile-time kernels (0: none), and the kernels that run
+// column lengths R and fast_cols_next(R) in one launch
+int fast_cols_next(int R);
+PassKernel fast_cols_pair_inv(int R);
+PassKernel fast_cols_pair_fwd(int R);
 // every kernel pointer, for raising the LDS limit once
 void all_kernels(PassKernel *out, int *count, int cap);
 

@@ -23,6 +23,22 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *slots, SlotDe
     cols_fwd_fast<R>(slots, one, jobs, mode, step);
 }
 
+// Two adjacent column lengths in one launch: tests of both shapes advance together (each work-
+// group runs the body of its test's R), so a batch straddling an FFT-length boundary fills the
+// device with one launch per column pass instead of two.
+template <int R1, int R2>
+__global__ void __launch_bounds__(kNT) k_cols_inv_rr(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    if (pick_slot(slots, one, jb.x).shp->R == R1) cols_inv_fast<R1>(slots, one, jobs, mode, step);
+    else cols_inv_fast<R2>(slots, one, jobs, mode, step);
+}
+template <int R1, int R2>
+__global__ void __launch_bounds__(kNT) k_cols_fwd_rr(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    if (pick_slot(slots, one, jb.x).shp->R == R1) cols_fwd_fast<R1>(slots, one, jobs, mode, step);
+    else cols_fwd_fast<R2>(slots, one, jobs, mode, step);
+}
+
 size_t fast_cols_lds(int R) {
     switch (R) {
 #define KITE_COLS_LDS(r) case r: return ColGeom<r>::lds_bytes();
@@ -132,6 +148,96 @@ PassKernel fast_cols_fwd(int R) {
     }
 }
 
+int fast_cols_next(int R) {
+    switch (R) {
+    case 8: return 12;
+    case 12: return 16;
+    case 16: return 20;
+    case 20: return 24;
+    case 24: return 28;
+    case 28: return 32;
+    case 32: return 40;
+    case 40: return 48;
+    case 48: return 56;
+    case 56: return 64;
+    case 64: return 80;
+    case 80: return 96;
+    case 96: return 112;
+    case 112: return 128;
+    case 128: return 160;
+    case 160: return 192;
+    case 192: return 224;
+    case 224: return 256;
+    case 256: return 320;
+    case 320: return 384;
+    case 384: return 448;
+    case 448: return 512;
+    case 512: return 640;
+    case 640: return 768;
+    default: return 0;
+    }
+}
+
+PassKernel fast_cols_pair_inv(int R1) {
+    switch (R1) {
+    case 8: return (PassKernel)k_cols_inv_rr<8, 12>;
+    case 12: return (PassKernel)k_cols_inv_rr<12, 16>;
+    case 16: return (PassKernel)k_cols_inv_rr<16, 20>;
+    case 20: return (PassKernel)k_cols_inv_rr<20, 24>;
+    case 24: return (PassKernel)k_cols_inv_rr<24, 28>;
+    case 28: return (PassKernel)k_cols_inv_rr<28, 32>;
+    case 32: return (PassKernel)k_cols_inv_rr<32, 40>;
+    case 40: return (PassKernel)k_cols_inv_rr<40, 48>;
+    case 48: return (PassKernel)k_cols_inv_rr<48, 56>;
+    case 56: return (PassKernel)k_cols_inv_rr<56, 64>;
+    case 64: return (PassKernel)k_cols_inv_rr<64, 80>;
+    case 80: return (PassKernel)k_cols_inv_rr<80, 96>;
+    case 96: return (PassKernel)k_cols_inv_rr<96, 112>;
+    case 112: return (PassKernel)k_cols_inv_rr<112, 128>;
+    case 128: return (PassKernel)k_cols_inv_rr<128, 160>;
+    case 160: return (PassKernel)k_cols_inv_rr<160, 192>;
+    case 192: return (PassKernel)k_cols_inv_rr<192, 224>;
+    case 224: return (PassKernel)k_cols_inv_rr<224, 256>;
+    case 256: return (PassKernel)k_cols_inv_rr<256, 320>;
+    case 320: return (PassKernel)k_cols_inv_rr<320, 384>;
+    case 384: return (PassKernel)k_cols_inv_rr<384, 448>;
+    case 448: return (PassKernel)k_cols_inv_rr<448, 512>;
+    case 512: return (PassKernel)k_cols_inv_rr<512, 640>;
+    case 640: return (PassKernel)k_cols_inv_rr<640, 768>;
+    default: return nullptr;
+    }
+}
+
+PassKernel fast_cols_pair_fwd(int R1) {
+    switch (R1) {
+    case 8: return (PassKernel)k_cols_fwd_rr<8, 12>;
+    case 12: return (PassKernel)k_cols_fwd_rr<12, 16>;
+    case 16: return (PassKernel)k_cols_fwd_rr<16, 20>;
+    case 20: return (PassKernel)k_cols_fwd_rr<20, 24>;
+    case 24: return (PassKernel)k_cols_fwd_rr<24, 28>;
+    case 28: return (PassKernel)k_cols_fwd_rr<28, 32>;
+    case 32: return (PassKernel)k_cols_fwd_rr<32, 40>;
+    case 40: return (PassKernel)k_cols_fwd_rr<40, 48>;
+    case 48: return (PassKernel)k_cols_fwd_rr<48, 56>;
+    case 56: return (PassKernel)k_cols_fwd_rr<56, 64>;
+    case 64: return (PassKernel)k_cols_fwd_rr<64, 80>;
+    case 80: return (PassKernel)k_cols_fwd_rr<80, 96>;
+    case 96: return (PassKernel)k_cols_fwd_rr<96, 112>;
+    case 112: return (PassKernel)k_cols_fwd_rr<112, 128>;
+    case 128: return (PassKernel)k_cols_fwd_rr<128, 160>;
+    case 160: return (PassKernel)k_cols_fwd_rr<160, 192>;
+    case 192: return (PassKernel)k_cols_fwd_rr<192, 224>;
+    case 224: return (PassKernel)k_cols_fwd_rr<224, 256>;
+    case 256: return (PassKernel)k_cols_fwd_rr<256, 320>;
+    case 320: return (PassKernel)k_cols_fwd_rr<320, 384>;
+    case 384: return (PassKernel)k_cols_fwd_rr<384, 448>;
+    case 448: return (PassKernel)k_cols_fwd_rr<448, 512>;
+    case 512: return (PassKernel)k_cols_fwd_rr<512, 640>;
+    case 640: return (PassKernel)k_cols_fwd_rr<640, 768>;
+    default: return nullptr;
+    }
+}
+
 void all_kernels(PassKernel *out, int *count, int cap) {
     int n = 0;
     PassKernel g[3] = {generic_rows(), generic_cols_inv(), generic_cols_fwd()};
@@ -143,6 +249,10 @@ void all_kernels(PassKernel *out, int *count, int cap) {
     for (int r : Rs) {
         if (n < cap) out[n++] = fast_cols_inv(r);
         if (n < cap) out[n++] = fast_cols_fwd(r);
+        if (fast_cols_next(r) && n + 1 < cap) {
+            out[n++] = fast_cols_pair_inv(r);
+            out[n++] = fast_cols_pair_fwd(r);
+        }
     }
     *count = n;
 }

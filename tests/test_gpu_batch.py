@@ -44,6 +44,33 @@ def test_mixed_lengths_one_launch():
         b.close()
 
 
+@pytest.mark.parametrize("cases", [
+    # the n~660K boundary: L 49152 (R 96) and 57344 (R 112) share one launch per column pass
+    [(8045, 659321, 0), (9001, 669990, 0), (3, 658000, 0), (9999, 669000, 0)],
+    # small adjacent column lengths from explicit FFT lengths
+    [(9357, 20000, 0), (4523, 20011, 2560), (3, 20000, 3072), (7, 19000, 3584)],
+])
+def test_adjacent_lengths_share_column_launch(cases):
+    """Tests whose column lengths are adjacent compile-time sizes run through the two-shape column
+    kernels; every value exact after several squarings."""
+    b = kite.Batch()
+    r = random.Random(11)
+    vals = {}
+    try:
+        for k, n, L in cases:
+            s = b.add(k, n, fft_len=L)
+            a = r.randrange(k * (1 << n) - 1)
+            b.set(s, 0, a)
+            vals[s] = (k, n, a)
+        say("lengths", sorted({b.fft_len(s) for s in vals}))
+        b.square(6)
+        for s, (k, n, a) in vals.items():
+            assert b.get(s, 0) == sqr_n(a, 6, k, n), (k, n, b.fft_len(s))
+        assert max(b.maxerr().values()) < 0.4
+    finally:
+        b.close()
+
+
 def test_batch_prp_continuous_admission_vs_gmp(golden):
     """The 16 seeded n~65K tests through 5 slots (tests admitted as others finish); RES64s equal
     GMP's, i.e. the same as each test alone."""
