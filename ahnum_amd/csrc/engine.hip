@@ -253,9 +253,12 @@ struct Engine {
     unsigned long long *maxerr = nullptr;
     std::vector<int32_t> hdig;
     int nrowjobs = 0, ncoljobs = 0;
-    size_t lds_rows = 0, lds_cols = 0;
+    size_t lds_rows = 0, lds_cols = 0, lds_fused = 0;
     PassKernel krows = nullptr, kcinv = nullptr, kcfwd = nullptr;
-    PassKernel krows_sq = nullptr;  // square-mode rows with registers (64-thread groups), if any
+    PassKernel krows_sq = nullptr;  // square-mode rows with registers, if any
+    PassKernel kcfused = nullptr;   // square-mode fused column pass, if any
+    unsigned long long *flags = nullptr;  // its per-chunk carry flags
+    uint32_t seq = 0;                     // its launches so far
     std::string name[3];  // kernel names of the three passes, for timing reports
     uint32_t H = 0;
 };
@@ -363,6 +366,8 @@ int engine_init(kite_ctx *ctx) {
     HIP_OK(hipMemcpy(E->dcode, dcode.data(), dcode.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMalloc(&E->dig, (size_t)ctx->L * 4));
     HIP_OK(hipMalloc(&E->cout, (size_t)sh.NG * sh.R * 8));
+    HIP_OK(hipMalloc(&E->flags, (size_t)sh.NG * sh.R * 8));
+    HIP_OK(hipMemset(E->flags, 0, (size_t)sh.NG * sh.R * 8));
     HIP_OK(hipMalloc(&E->yfreq, (size_t)E->H * sizeof(double2)));
     HIP_OK(hipMalloc(&E->maxerr, 16));
     HIP_OK(hipMemset(E->maxerr, 0, 16));
@@ -373,6 +378,8 @@ int engine_init(kite_ctx *ctx) {
     s.dcode = E->dcode;
     s.dig = E->dig;
     s.cout = E->cout;
+    s.flags = E->flags;
+    s.seq = 0;
     s.yfreq = E->yfreq;
     s.maxerr = E->maxerr;
     s.count = 1;
@@ -390,6 +397,8 @@ int engine_init(kite_ctx *ctx) {
         E->krows_sq = (fr && ppw_for_C(sh.C) == 4) ? fast_rows_sq(sh.C) : nullptr;
         E->kcinv = fi ? fi : generic_cols_inv();
         E->kcfwd = ff ? ff : generic_cols_fwd();
+        E->kcfused = ff ? fast_cols_fused(sh.R) : nullptr;
+        E->lds_fused = ff ? fast_cols_fused_lds(sh.R) : 0;
         E->name[0] = E->krows_sq ? "k_rows_sq" + std::to_string(sh.C)
                      : fr       ? "k_rows_c<" + std::to_string(sh.C) + ">"
                                 : std::string("k_rows_g");
@@ -416,7 +425,7 @@ void engine_free(kite_ctx *ctx) {
     Engine *E = ctx->eng;
     if (!E) return;
     if (E->stream) hipStreamSynchronize(E->stream);
-    void *ps[] = {E->ta, E->tb, E->dcode, E->dig, E->cout, E->yfreq, E->maxerr};
+    void *ps[] = {E->ta, E->tb, E->dcode, E->dig, E->cout, E->flags, E->yfreq, E->maxerr};
     for (void *p : ps)
         if (p) hipFree(p);
     if (E->stream) hipStreamDestroy(E->stream);
@@ -468,7 +477,14 @@ void engine_square(kite_ctx *ctx, void *x, uint64_t count) {
     Engine *E = ctx->eng;
     for (uint64_t i = 0; i < count; i++) {
         launch_rows(E, (double2 *)x, nullptr, kModeSquare);
-        launch_cols(E, (double2 *)x, kModeSquare);
+        if (E->kcfused) {
+            SlotDev s = E->slot;
+            s.x = (double2 *)x;
+            s.seq = E->seq++;
+            launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, nullptr, s, nullptr, kModeSquare, 0);
+        } else {
+            launch_cols(E, (double2 *)x, kModeSquare);
+        }
     }
 }
 
@@ -545,6 +561,7 @@ namespace kite {
 struct BatchSlot {
     kite_ctx *ctx = nullptr;
     std::vector<kite_num *> nums;
+    uint32_t seq = 0;  // fused column launches so far
 };
 
 }  // namespace kite
@@ -563,6 +580,9 @@ struct kite_batch {
         size_t lds;
         double sumL;        // sum of the FFT lengths of the group's tests
         int stat[2];        // index into kstats of k1, k2
+        kite::PassKernel kf = nullptr;  // columns: square-mode fused pass, if any
+        int stat_f = -1;
+        size_t lds_f = 0;
     };
     std::vector<Group> grows, gcols;
     unsigned long long *d_maxerr = nullptr;
@@ -641,6 +661,7 @@ static int batch_refresh(kite_batch *b) {
                 const int nj = isrows ? E->nrowjobs : E->ncoljobs;
                 for (int u = 0; u < nj; u++) jobs.push_back(make_int2(s, u));
                 g.lds = std::max(g.lds, isrows ? E->lds_rows : E->lds_cols);
+                if (!isrows) g.lds_f = std::max(g.lds_f, E->lds_fused);
                 g.sumL += 2.0 * E->H;
             }
             g.count = (int)jobs.size() - g.off;
@@ -649,6 +670,8 @@ static int batch_refresh(kite_batch *b) {
             } else {
                 g.stat[0] = stat_of(E0->name[1], 1);
                 g.stat[1] = stat_of(E0->name[2], 2);
+                g.kf = E0->kcfused;
+                if (g.kf) g.stat_f = stat_of("k_cols_fused_r<" + std::to_string(E0->shape->ch.R) + ">", 1);
             }
             grp.push_back(g);
             gR.push_back((int)E0->shape->ch.R);
@@ -665,10 +688,13 @@ static int batch_refresh(kite_batch *b) {
                 g.k2 = fast_cols_pair_fwd(r1);
                 g.count += grp[i + 1].count;
                 g.lds = std::max(g.lds, grp[i + 1].lds);
+                g.lds_f = std::max(g.lds_f, grp[i + 1].lds_f);
                 g.sumL += grp[i + 1].sumL;
                 const std::string rr = "<" + std::to_string(r1) + "," + std::to_string(gR[i + 1]) + ">";
                 g.stat[0] = stat_of("k_cols_inv_rr" + rr, 1);
                 g.stat[1] = stat_of("k_cols_fwd_rr" + rr, 2);
+                g.kf = (grp[i].kf && grp[i + 1].kf) ? fast_cols_pair_fused(r1) : nullptr;
+                g.stat_f = g.kf ? stat_of("k_cols_fused_rr" + rr, 1) : -1;
                 merged.push_back(g);
                 i++;
             } else {
@@ -711,6 +737,7 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
         d.maxerr = b->d_maxerr + s;
         const bool on = xi[s] >= 0 && (size_t)xi[s] < b->slots[s].nums.size() && count[s] > 0;
         d.count = on ? count[s] : 0;
+        d.seq = b->slots[s].seq;
         d.x = on ? (double2 *)b->slots[s].nums[xi[s]]->dev : nullptr;
         d.y_src = (on && yi && yi[s] >= 0) ? (const double2 *)b->slots[s].nums[yi[s]]->dev : nullptr;
     }
@@ -720,6 +747,7 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
 }
 
 // One launch per group of the pass; with `timed`, each launch is bracketed by an event pair.
+// In square mode a column group with a fused kernel runs it as pass 1 and skips pass 2.
 static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed = false) {
     SlotDev none{};
     const auto &groups = pass == 0 ? b->grows : b->gcols;
@@ -727,8 +755,10 @@ static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed
     for (const auto &g : groups) {
         if (!g.count) continue;
         const bool rsq = pass == 0 && mode == kModeSquare && g.k2;
-        const PassKernel fn = (pass == 2 || rsq) ? g.k2 : g.k1;
-        const size_t lds = rsq ? 0 : g.lds;
+        const bool fused = pass > 0 && mode == kModeSquare && g.kf;
+        if (fused && pass == 2) continue;
+        const PassKernel fn = fused ? g.kf : (pass == 2 || rsq) ? g.k2 : g.k1;
+        const size_t lds = rsq ? 0 : fused ? g.lds_f : g.lds;
         const int threads = rsq ? kRowsSqThreads : kNT;
         if (timed) {
             const size_t e = 2 * b->ev_stat.size();
@@ -737,7 +767,7 @@ static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed
                 hipEventCreate(&b->ev[e]);
                 hipEventCreate(&b->ev[e + 1]);
             }
-            b->ev_stat.push_back(g.stat[pass == 2 ? 1 : 0]);
+            b->ev_stat.push_back(fused ? g.stat_f : g.stat[pass == 2 ? 1 : 0]);
             hipEventRecord(b->ev[e], b->stream);
             launch_pass(fn, g.count, lds, b->stream, b->d_slots, none, jobs + g.off, mode, step, threads);
             hipEventRecord(b->ev[e + 1], b->stream);
@@ -833,6 +863,8 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
     if (!maxc) return 0;
     if (batch_upload(b, which, nullptr, count)) return -1;
     b->ev_stat.clear();
+    for (size_t s = 0; s < b->slots.size(); s++)
+        if (b->slots[s].ctx) b->slots[s].seq += b->h_slots[s].count;  // the uploaded table holds the old value
     for (uint32_t st = 0; st < maxc; st++) {
         const bool timed = b->timing && st % (uint32_t)b->timing == 0;
         batch_launch(b, 0, kModeSquare, (int)st, timed);
@@ -855,6 +887,7 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
             if (g.count) {
                 bytes[g.stat[0]] += 8.0 * g.sumL;
                 bytes[g.stat[1]] += 8.0 * g.sumL;
+                if (g.kf) bytes[g.stat_f] += 16.0 * g.sumL;
             }
         for (size_t i = 0; i < b->ev_stat.size(); i++) {
             float ms = 0;

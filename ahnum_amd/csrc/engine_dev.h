@@ -83,6 +83,8 @@ struct SlotDev {  // one number being operated on (one test in a batch)
     const double *ta;           // [R] 2^-fa(r), [R] 2^+fa(r)
     const double *tb;           // [2C] 2^-fb(b), [2C] 2^+fb(b)
     unsigned long long *maxerr; // max round-off (bits of a non-negative double)
+    unsigned long long *flags;  // fused column pass: per (g, r) chunk, seq << 40 | carry (40 bits)
+    uint32_t seq;               // fused column launches of this slot so far (step s uses seq + s + 1)
     double kscale;              // k / (4H)
     double rscale;              // 1 / R
     uint64_t k;

@@ -1097,6 +1097,169 @@ __device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDe
     }
 }
 
+// Fused column pass (square mode): cols_inv_fast and cols_fwd_fast in one work-group, the digits
+// kept in registers. The carry out of each row chunk (g, r) goes to the work-group of the next
+// chunk through flags[g R + r] = seq << 40 | carry (low 40 bits, two's complement), written and
+// polled with agent-scope atomics: a single 64-bit word holds both, so no fence is needed. The
+// work-group of chunk (g, r) waits for chunk (g-1, r) (g > 0) or (NG-1, r-1) (g = 0; (NG-1, R-1)
+// holds the fold); every work-group publishes before it waits, and work-groups are dispatched in
+// order, so every wait ends. A wait is bounded: past kSpinLimit polls the slot's round-off is set
+// to infinity (the host reports the failure) and the carry is taken as 0.
+constexpr uint32_t kSeqMask = 0xFFFFFF;
+constexpr int kSpinLimit = 1 << 22;
+
+__device__ __forceinline__ unsigned long long pack_flag(uint32_t seq, int64_t carry) {
+    return ((unsigned long long)(seq & kSeqMask) << 40) | ((unsigned long long)carry & ((1ull << 40) - 1));
+}
+
+__device__ __forceinline__ int64_t wait_flag(unsigned long long *f, uint32_t seq, unsigned long long *maxerr) {
+    const unsigned long long want = (unsigned long long)(seq & kSeqMask);
+    for (int spin = 0;; spin++) {
+        const unsigned long long v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((v >> 40) == want) return (int64_t)(v << 24) >> 24;  // sign-extend the 40-bit carry
+        if (spin >= kSpinLimit) {
+            atomicMax(maxerr, 0x7FF0000000000000ull);  // +inf
+            return 0;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Balanced split by exponents: x an integer with |x| < 2^52, b its digit width; x 2^-b, + 1/2,
+// floor and x - q 2^b are exact. Returns the carry q, sets the digit d.
+__device__ __forceinline__ double bal_e(double x, int b, double &d) {
+    const double q = floor(ldexp(x, -b) + 0.5);
+    d = x - ldexp(q, b);
+    return q;
+}
+
+template <int R> struct FusedGeom {
+    static constexpr int G = g_for_R(R), GP = G + 1, SUB = G < kSub ? G : kSub, NSUB = G / SUB;
+    static constexpr int NE = R * G, NSC = R * NSUB, ITSC = (NSC + kNT - 1) / kNT, ITE = (NE + kNT - 1) / kNT;
+    // LDS: R x G elements with row stride GP (the padding spreads the sub-chunk reads over the
+    // banks), the column twiddles, one carry per sub-chunk, a 64-digit scratch for the fold
+    static constexpr size_t lds_bytes() { return (size_t)16 * (R * GP + R) + (size_t)8 * NSC + 4 * 64; }
+};
+
+template <int R>
+__device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs,
+                                                int step) {
+    extern __shared__ double2 lds[];
+    using Gm = FusedGeom<R>;
+    constexpr int G = Gm::G, GP = Gm::GP, SUB = Gm::SUB, NSUB = Gm::NSUB, NE = Gm::NE, NSC = Gm::NSC;
+    constexpr int ITSC = Gm::ITSC, ITE = Gm::ITE;
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = pick_slot(slots, one, jb.x);
+    if (step >= (int)s.count) return;
+    const ShapeDev &sh = *s.shp;
+    const int C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    const uint32_t seq = s.seq + (uint32_t)step + 1;
+    double2 *A = lds, *T = lds + R * GP;
+    int64_t *SC = reinterpret_cast<int64_t *>(T + R);
+    int32_t *FD = reinterpret_cast<int32_t *>(SC + NSC);
+    for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+    const double2 *tg = sh.twCols + (size_t)g * NE;
+#pragma unroll
+    for (int it = 0; it < ITE; it++) {
+        const int i = it * kNT + (int)threadIdx.x;
+        if (NE % kNT != 0 && i >= NE) continue;
+        const int kr = i / G, l = i - kr * G;
+        A[kr * GP + l] = cmulc(gld(s.x + (size_t)kr * C + c0 + l), gld(tg + i));
+    }
+    __syncthreads();
+    fft_ip<R, 1, 0, G, GP, 1, true, true>(A, T);
+    const int b0 = s.b0;
+    const uint8_t *code = s.dcode + (size_t)g * R * G;
+    const double *tbi = s.tb + 2 * C + 2 * c0;
+    double lmax = 0;
+    int32_t d[ITSC][2 * SUB];
+    uint32_t bg[ITSC];  // bit i: digit i of the sub-chunk has b0+1 bits
+#pragma unroll
+    for (int it = 0; it < ITSC; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int r = w / NSUB, q = w - r * NSUB;
+        const double tap = s.ta[R + r] * s.kscale;
+        double cy = 0;
+        bg[it] = 0;
+#pragma unroll
+        for (int l = 0; l < SUB; l++) {
+            const int col = q * SUB + l;
+            const double2 v = A[r * GP + col];
+            const int cd = code[r * G + col];
+            const double x0 = ldexp(v.x * (tap * tbi[2 * col]), -(cd & 3));
+            const double x1 = ldexp(v.y * (tap * tbi[2 * col + 1]), -((cd >> 3) & 3));
+            const double r0 = rint(x0), r1 = rint(x1);
+            lmax = fmax(lmax, fmax(fabs(x0 - r0), fabs(x1 - r1)));
+            const int b0f = (cd >> 2) & 1, b1f = (cd >> 5) & 1;
+            bg[it] |= (uint32_t)(b0f | (b1f << 1)) << (2 * l);
+            double dd;
+            cy = bal_e(r0 + cy, b0 + b0f, dd);
+            d[it][2 * l] = (int32_t)dd;
+            cy = bal_e(r1 + cy, b0 + b1f, dd);
+            d[it][2 * l + 1] = (int32_t)dd;
+        }
+        int64_t co = (int64_t)cy;
+        if (g == NG - 1 && w == NSC - 1) {  // the top fold on the number's last sub-chunk
+#pragma unroll
+            for (int i = 0; i < 2 * SUB; i++) FD[i] = d[it][i];
+            co = fold_top_dev(s, FD, co);
+#pragma unroll
+            for (int i = 0; i < 2 * SUB; i++) d[it][i] = FD[i];
+        }
+        SC[w] = co;
+        if (q == NSUB - 1)  // this row chunk's carry, for the next chunk's work-group
+            __hip_atomic_store(s.flags + g * R + r, pack_flag(seq, co), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    // absorb the left neighbour's carry (within the work-group, or the previous chunk's through its
+    // flag); the last digit keeps the rest. Then weight into A for the forward column FFT.
+    const double *tbf = s.tb + 2 * c0;
+#pragma unroll
+    for (int it = 0; it < ITSC; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int r = w / NSUB, q = w - r * NSUB;
+        int64_t cy;
+        if (q > 0) {
+            cy = SC[w - 1];
+        } else {
+            const int pidx = g > 0 ? (g - 1) * R + r : (r > 0 ? (NG - 1) * R + r - 1 : (NG - 1) * R + R - 1);
+            cy = wait_flag(s.flags + pidx, seq, s.maxerr);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * SUB; i++) {
+            if (!cy) break;
+            if (i == 2 * SUB - 1) {
+                d[it][i] = (int32_t)(d[it][i] + cy);
+                break;
+            }
+            cy = bal((int64_t)d[it][i] + cy, b0 + (int)((bg[it] >> i) & 1), d[it][i]);
+        }
+        const double tam = s.ta[r];
+#pragma unroll
+        for (int l = 0; l < SUB; l++) {
+            const int col = q * SUB + l;
+            const int cd = code[r * G + col];
+            const double w0 = ldexp(tam * tbf[2 * col], cd & 3);
+            const double w1 = ldexp(tam * tbf[2 * col + 1], (cd >> 3) & 3);
+            A[r * GP + col] = make_double2((double)d[it][2 * l] * w0, (double)d[it][2 * l + 1] * w1);
+        }
+    }
+    __syncthreads();
+    fft_ip<R, 1, 0, G, GP, 1, true, false>(A, T);
+#pragma unroll
+    for (int it = 0; it < ITE; it++) {
+        const int i = it * kNT + (int)threadIdx.x;
+        if (NE % kNT != 0 && i >= NE) continue;
+        const int kr = i / G, l = i - kr * G;
+        gst(s.x + (size_t)kr * C + c0 + l, cmul(A[kr * GP + l], gld(tg + i)));
+    }
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+    if ((threadIdx.x & 63) == 0 && lmax > 0)
+        atomicMax(s.maxerr, (unsigned long long)__double_as_longlong(lmax));
+}
+
 }  // namespace kite
 
 #endif

@@ -28,6 +28,10 @@ size_t fast_cols_lds(int R);  // LDS bytes of the compile-time column kernels
 int fast_cols_next(int R);
 PassKernel fast_cols_pair_inv(int R);
 PassKernel fast_cols_pair_fwd(int R);
+// square mode: both column passes in one kernel, carries between work-groups through flags
+PassKernel fast_cols_fused(int R);
+size_t fast_cols_fused_lds(int R);  // LDS bytes of the fused column kernel
+PassKernel fast_cols_pair_fused(int R);
 // every kernel pointer, for raising the LDS limit once
 void all_kernels(PassKernel *out, int *count, int cap);
 

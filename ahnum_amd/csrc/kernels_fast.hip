@@ -23,6 +23,12 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *slots, SlotDe
     cols_fwd_fast<R>(slots, one, jobs, mode, step);
 }
 
+template <int R>
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_r(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    (void)mode;
+    cols_fused_fast<R>(slots, one, jobs, step);
+}
+
 // Two adjacent column lengths in one launch: tests of both shapes advance together (each work-
 // group runs the body of its test's R), so a batch straddling an FFT-length boundary fills the
 // device with one launch per column pass instead of two.
@@ -37,6 +43,27 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_rr(const SlotDev *slots, SlotD
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     if (pick_slot(slots, one, jb.x).shp->R == R1) cols_fwd_fast<R1>(slots, one, jobs, mode, step);
     else cols_fwd_fast<R2>(slots, one, jobs, mode, step);
+}
+
+template <int R1, int R2>
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    (void)mode;
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    if (pick_slot(slots, one, jb.x).shp->R == R1) cols_fused_fast<R1>(slots, one, jobs, step);
+    else cols_fused_fast<R2>(slots, one, jobs, step);
+}
+
+size_t fast_cols_fused_lds(int R) {
+    switch (R) {
+#define KITE_FUSED_LDS(r) case r: return FusedGeom<r>::lds_bytes();
+    KITE_FUSED_LDS(8) KITE_FUSED_LDS(12) KITE_FUSED_LDS(16) KITE_FUSED_LDS(20) KITE_FUSED_LDS(24)
+    KITE_FUSED_LDS(28) KITE_FUSED_LDS(32) KITE_FUSED_LDS(40) KITE_FUSED_LDS(48) KITE_FUSED_LDS(56)
+    KITE_FUSED_LDS(64) KITE_FUSED_LDS(80) KITE_FUSED_LDS(96) KITE_FUSED_LDS(112) KITE_FUSED_LDS(128)
+    KITE_FUSED_LDS(160) KITE_FUSED_LDS(192) KITE_FUSED_LDS(224) KITE_FUSED_LDS(256) KITE_FUSED_LDS(320)
+    KITE_FUSED_LDS(384) KITE_FUSED_LDS(448) KITE_FUSED_LDS(512) KITE_FUSED_LDS(640) KITE_FUSED_LDS(768)
+#undef KITE_FUSED_LDS
+    default: return 0;
+    }
 }
 
 size_t fast_cols_lds(int R) {
@@ -238,6 +265,67 @@ PassKernel fast_cols_pair_fwd(int R1) {
     }
 }
 
+PassKernel fast_cols_fused(int R) {
+    switch (R) {
+    case 8: return (PassKernel)k_cols_fused_r<8>;
+    case 12: return (PassKernel)k_cols_fused_r<12>;
+    case 16: return (PassKernel)k_cols_fused_r<16>;
+    case 20: return (PassKernel)k_cols_fused_r<20>;
+    case 24: return (PassKernel)k_cols_fused_r<24>;
+    case 28: return (PassKernel)k_cols_fused_r<28>;
+    case 32: return (PassKernel)k_cols_fused_r<32>;
+    case 40: return (PassKernel)k_cols_fused_r<40>;
+    case 48: return (PassKernel)k_cols_fused_r<48>;
+    case 56: return (PassKernel)k_cols_fused_r<56>;
+    case 64: return (PassKernel)k_cols_fused_r<64>;
+    case 80: return (PassKernel)k_cols_fused_r<80>;
+    case 96: return (PassKernel)k_cols_fused_r<96>;
+    case 112: return (PassKernel)k_cols_fused_r<112>;
+    case 128: return (PassKernel)k_cols_fused_r<128>;
+    case 160: return (PassKernel)k_cols_fused_r<160>;
+    case 192: return (PassKernel)k_cols_fused_r<192>;
+    case 224: return (PassKernel)k_cols_fused_r<224>;
+    case 256: return (PassKernel)k_cols_fused_r<256>;
+    case 320: return (PassKernel)k_cols_fused_r<320>;
+    case 384: return (PassKernel)k_cols_fused_r<384>;
+    case 448: return (PassKernel)k_cols_fused_r<448>;
+    case 512: return (PassKernel)k_cols_fused_r<512>;
+    case 640: return (PassKernel)k_cols_fused_r<640>;
+    case 768: return (PassKernel)k_cols_fused_r<768>;
+    default: return nullptr;
+    }
+}
+
+PassKernel fast_cols_pair_fused(int R1) {
+    switch (R1) {
+    case 8: return (PassKernel)k_cols_fused_rr<8, 12>;
+    case 12: return (PassKernel)k_cols_fused_rr<12, 16>;
+    case 16: return (PassKernel)k_cols_fused_rr<16, 20>;
+    case 20: return (PassKernel)k_cols_fused_rr<20, 24>;
+    case 24: return (PassKernel)k_cols_fused_rr<24, 28>;
+    case 28: return (PassKernel)k_cols_fused_rr<28, 32>;
+    case 32: return (PassKernel)k_cols_fused_rr<32, 40>;
+    case 40: return (PassKernel)k_cols_fused_rr<40, 48>;
+    case 48: return (PassKernel)k_cols_fused_rr<48, 56>;
+    case 56: return (PassKernel)k_cols_fused_rr<56, 64>;
+    case 64: return (PassKernel)k_cols_fused_rr<64, 80>;
+    case 80: return (PassKernel)k_cols_fused_rr<80, 96>;
+    case 96: return (PassKernel)k_cols_fused_rr<96, 112>;
+    case 112: return (PassKernel)k_cols_fused_rr<112, 128>;
+    case 128: return (PassKernel)k_cols_fused_rr<128, 160>;
+    case 160: return (PassKernel)k_cols_fused_rr<160, 192>;
+    case 192: return (PassKernel)k_cols_fused_rr<192, 224>;
+    case 224: return (PassKernel)k_cols_fused_rr<224, 256>;
+    case 256: return (PassKernel)k_cols_fused_rr<256, 320>;
+    case 320: return (PassKernel)k_cols_fused_rr<320, 384>;
+    case 384: return (PassKernel)k_cols_fused_rr<384, 448>;
+    case 448: return (PassKernel)k_cols_fused_rr<448, 512>;
+    case 512: return (PassKernel)k_cols_fused_rr<512, 640>;
+    case 640: return (PassKernel)k_cols_fused_rr<640, 768>;
+    default: return nullptr;
+    }
+}
+
 void all_kernels(PassKernel *out, int *count, int cap) {
     int n = 0;
     PassKernel g[3] = {generic_rows(), generic_cols_inv(), generic_cols_fwd()};
@@ -249,9 +337,11 @@ void all_kernels(PassKernel *out, int *count, int cap) {
     for (int r : Rs) {
         if (n < cap) out[n++] = fast_cols_inv(r);
         if (n < cap) out[n++] = fast_cols_fwd(r);
-        if (fast_cols_next(r) && n + 1 < cap) {
+        if (n < cap) out[n++] = fast_cols_fused(r);
+        if (fast_cols_next(r) && n + 2 < cap) {
             out[n++] = fast_cols_pair_inv(r);
             out[n++] = fast_cols_pair_fwd(r);
+            out[n++] = fast_cols_pair_fused(r);
         }
     }
     *count = n;

@@ -7,7 +7,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libkite.so")
+LIB_PATH = os.environ.get("KITE_LIB") or os.path.join(HERE, "lib", "libkite.so")  # KITE_LIB: another build of the library
 BIN_PATH = os.path.join(HERE, "bin", "kite")
 
 c_u64 = ctypes.c_uint64
