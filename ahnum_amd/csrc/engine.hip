@@ -219,10 +219,17 @@ static ShapeEntry *get_shape(const ShapeChoice &ch) {
         for (uint64_t kr = 0; kr < ch.R; kr++)
             for (uint64_t l = 0; l < ch.G; l++)
                 tcols[(g * ch.R + kr) * ch.G + l] = wH[((g * ch.G + l) * kr) % H];
-    double2 *twR, *twC, *tlo, *thi, *twRows, *twCols;
+    // 16 x 16 row FFTs: [j][t] = w_C^{t j}
+    std::vector<double2> t16(256, make_double2(1, 0));
+    if (ch.C == 256) {
+        const std::vector<double2> wC = roots_table(ch.C, ch.C, 1);
+        for (uint64_t j = 0; j < 16; j++)
+            for (uint64_t t = 0; t < 16; t++) t16[j * 16 + t] = wC[(t * j) % ch.C];
+    }
+    double2 *twR, *twC, *tlo, *thi, *twRows, *twCols, *tw16;
     if (upload(roots_table(ch.R, ch.R, 1), &twR, e->bufs) || upload(roots_table(ch.C, ch.C, 1), &twC, e->bufs) ||
         upload(roots_table(H, S, 1), &tlo, e->bufs) || upload(roots_table(H, nhi, S), &thi, e->bufs) ||
-        upload(trows, &twRows, e->bufs) || upload(tcols, &twCols, e->bufs)) {
+        upload(trows, &twRows, e->bufs) || upload(tcols, &twCols, e->bufs) || upload(t16, &tw16, e->bufs)) {
         delete e;
         return nullptr;
     }
@@ -232,6 +239,7 @@ static ShapeEntry *get_shape(const ShapeChoice &ch) {
     h.thi = thi;
     h.twRows = twRows;
     h.twCols = twCols;
+    h.tw16 = tw16;
     std::vector<ShapeDev> one(1, h);
     if (upload(one, &e->dev, e->bufs)) {
         delete e;

@@ -70,6 +70,7 @@ struct ShapeDev {  // one (R, C, G) shape: H = R*C complex, NG = C/G column grou
     const double2 *thi;    // w_H^(t 2^sshift)
     const double2 *twRows; // [k_r][k_c] = w_H^(k_r + R k_c): split twiddles in row order
     const double2 *twCols; // [g][k_r][l] = w_H^((g G + l) k_r): four-step twiddles in group order
+    const double2 *tw16;   // C = 256: [j][t] = w_C^(t j), j, t < 16 (the 16 x 16 row FFT's twiddles)
 };
 
 struct SlotDev {  // one number being operated on (one test in a batch)

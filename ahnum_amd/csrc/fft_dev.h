@@ -830,19 +830,59 @@ __device__ __forceinline__ double2 split_sq_half(double2 u, double2 w, double2 t
 
 // w_H^{p + R k_c} = T0 w16^{k2} for side a's k_c = t + 16 k2 (H = 256 R, T0 = w_H^{p + R t}); the
 // side-b element k2 pairs with side a's k2' = 15 - k2 of the same t.
-template <int K2, bool SIDE_B>
-__device__ __forceinline__ void rows256_split_step(double2 *v, const double2 *Xp, int km, double2 T0) {
-    if constexpr (K2 < 16) {
-        const int kc = km + 16 * K2;
-        v[K2] = split_sq_half(v[K2], Xp[kRowsRegC - 1 - kc], mul_w16<SIDE_B ? 15 - K2 : K2, false>(T0), SIDE_B);
-        if constexpr (K2 % 4 == 3) __builtin_amdgcn_sched_barrier(0);  // bounds the hoisting of LDS reads
-        rows256_split_step<K2 + 1, SIDE_B>(v, Xp, km, T0);
+// One round of the real split for this lane's elements k2 in [K0, K0 + 8) (pairs k_r, R - k_r with
+// k_r > 0): the partner of k_c = km + 16 k2 is the partner row's element C - 1 - k_c, which the
+// partner group has just written to its LDS block at natural position mod 128. Results go to
+// out[k2 - K0]; side a's twiddle w_H^{k_r + R k_c} is T0 w16^{k2} (T0 = w_H^{k_r + R t}).
+template <int K0, int K2, bool SIDE_B>
+__device__ __forceinline__ void rows256_split_half(const double2 *v, double2 *out, const double2 *Xp, int km,
+                                                   double2 T0) {
+    if constexpr (K2 < K0 + 8) {
+        const double2 w = Xp[(kRowsRegC - 1 - km - 16 * K2) & 127];
+        out[K2 - K0] = split_sq_half(v[K2], w, mul_w16<SIDE_B ? 15 - K2 : K2, false>(T0), SIDE_B);
+        rows256_split_half<K0, K2 + 1, SIDE_B>(v, out, Xp, km, T0);
     }
+}
+
+// The same round for the first four pairs, where rows 0 and R/2 pair within themselves
+// (k_c <-> (C - k_c) mod C and k_c <-> C - 1 - k_c): exact split twiddles from the table.
+template <int K0>
+__device__ __forceinline__ void rows256_split_half0(const double2 *v, double2 *out, const double2 *blk, int km,
+                                                    bool special, int side, int row, int p,
+                                                    const double2 *twRows) {
+    constexpr int C = kRowsRegC;
+#pragma unroll
+    for (int k2 = K0; k2 < K0 + 8; k2++) {
+        const int kc = km + 16 * k2;
+        const int pb = special ? (side == 0 ? (C - kc) & (C - 1) : C - 1 - kc) : C - 1 - kc;
+        const bool isb = special ? kc > pb : side == 1;
+        const int twi = (special ? row : p) * C + (isb ? pb : kc);
+        const double2 w = pb == kc ? v[k2] : blk[pb & 127];
+        out[k2 - K0] = split_sq_half(v[k2], w, gld(twRows + twi), isb);
+    }
+}
+
+// Transpose of 16 values per lane through a group's LDS block of 256 doubles, real and imaginary
+// parts in turn: value j of this lane goes to slot wr(j), and value j comes back from slot rd(j).
+template <class WR, class RD>
+__device__ __forceinline__ void xpose_ri(double2 *v, double *X, WR wr, RD rd) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) X[wr(j)] = v[j].x;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j].x = X[rd(j)];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) X[wr(j)] = v[j].y;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j].y = X[rd(j)];
+    __syncthreads();
 }
 
 __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int step) {
     constexpr int C = kRowsRegC;
-    __shared__ double2 xs[8 * 256];
+    __shared__ double xs[8 * 256];  // one 2 KB block per 16-lane group
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     const SlotDev &s = pick_slot(slots, one, jb.x);
     if (step >= (int)s.count) return;
@@ -855,53 +895,63 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &
     if (p < npairs) row = side == 0 ? p : (special ? (R % 2 == 0 ? R / 2 : -1) : R - p);
     const bool act = row >= 0;
     const int km = (side == 1 && !special) ? 15 - t : t;  // k1 held after the transpose
-    double2 *X = xs + g * 256;
+    double *X = xs + g * 256;
     double2 *xr = s.x + (size_t)(act ? row : 0) * C + t;
-    const double2 *twC = sh.twC;
+    const double2 *tw16 = sh.tw16;
     double2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = act ? gld(xr + 16 * i) : make_double2(0, 0);
     // forward: DFT16 over a, twiddle w_256^{t k1}, transpose, DFT16 over b
     Dft<16, false>::run(v);
+    // twiddles w_256^{t j} from the [j][t] table, four loads in flight at a time (register pressure)
 #pragma unroll
-    for (int j = 1; j < 16; j++) v[j] = cmul(v[j], gld(twC + t * j));
-#pragma unroll
-    for (int j = 0; j < 16; j++) X[j * 16 + (t ^ j)] = v[j];
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = X[km * 16 + (j ^ km)];
-    __syncthreads();
+    for (int j = 1; j < 16; j++) {
+        if (j % 4 == 1) __builtin_amdgcn_sched_barrier(0);
+        v[j] = cmul(v[j], gld(tw16 + 16 * j + t));
+    }
+    xpose_ri(v, X, [&](int j) { return j * 16 + (t ^ j); }, [&](int j) { return km * 16 + (j ^ km); });
     Dft<16, false>::run(v);  // v[k2] = U[row][km + 16 k2]
-    // real-split square against the partners read from LDS
+    // real-split square: the partner of element k2 (k_c = km + 16 k2) is the partner row's element
+    // 15 - k2 held by lane t of the partner group (whose km is 15 - km); two rounds of 8 through LDS
+    // real-split square in two rounds through LDS: every partner of an element with k_c < 128 has
+    // k_c >= 128 and the other way round (or is the element itself), so each group first publishes
+    // its upper half (k2 = 8..15) and combines its lower half, then publishes its lower half and
+    // combines its upper half in place
+    double2 *Xc = reinterpret_cast<double2 *>(X);
+    const bool first = jb.y == 0;  // uniform over the work-group: holds rows 0 and R/2
+    const double2 *blk = (first && special) ? Xc : reinterpret_cast<const double2 *>(xs + (g ^ 4) * 256);
+    const double2 T0 = (act && !first) ? gld(sh.twRows + (size_t)p * C + t) : make_double2(0, 0);
+    double2 lo[8];
 #pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) X[km + 16 * k2] = v[k2];
+    for (int k2 = 8; k2 < 16; k2++) Xc[(km + 16 * k2) & 127] = v[k2];
     __syncthreads();
-    if (act) {
-        if (!special) {
-            const double2 T0 = gld(sh.twRows + (size_t)p * C + t);
-            if (side == 0) rows256_split_step<0, false>(v, xs + (g ^ 4) * 256, km, T0);
-            else rows256_split_step<0, true>(v, xs + (g ^ 4) * 256, km, T0);
-        } else {
-            const double2 *tw = sh.twRows + (size_t)row * C;
-#pragma unroll
-            for (int k2 = 0; k2 < 16; k2++) {
-                const int kc = t + 16 * k2, pb = side == 0 ? (C - kc) & (C - 1) : C - 1 - kc;
-                const bool isb = kc > pb;
-                v[k2] = split_sq_half(v[k2], X[pb], gld(tw + (isb ? pb : kc)), isb);
-                if (k2 % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-            }
-        }
+    if (!first) {
+        if (side == 0) rows256_split_half<0, 0, false>(v, lo, blk, km, T0);
+        else rows256_split_half<0, 0, true>(v, lo, blk, km, T0);
+    } else if (act) {
+        rows256_split_half0<0>(v, lo, blk, km, special, side, row, p, sh.twRows);
     }
     __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) Xc[km + 16 * k2] = v[k2];
+    __syncthreads();
+    if (!first) {
+        if (side == 0) rows256_split_half<8, 8, false>(v, v + 8, blk, km, T0);
+        else rows256_split_half<8, 8, true>(v, v + 8, blk, km, T0);
+    } else if (act) {
+        rows256_split_half0<8>(v, v + 8, blk, km, special, side, row, p, sh.twRows);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) v[k2] = lo[k2];
     // inverse: DFT16 over k2, twiddle w_256^{-k1 b}, transpose, DFT16 over k1
     Dft<16, true>::run(v);
 #pragma unroll
-    for (int j = 1; j < 16; j++) v[j] = cmulc(v[j], gld(twC + km * j));
-#pragma unroll
-    for (int j = 0; j < 16; j++) X[j * 16 + (km ^ j)] = v[j];
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = X[t * 16 + (j ^ t)];
+    for (int j = 1; j < 16; j++) {
+        if (j % 4 == 1) __builtin_amdgcn_sched_barrier(0);
+        v[j] = cmulc(v[j], gld(tw16 + 16 * j + km));
+    }
+    xpose_ri(v, X, [&](int j) { return j * 16 + (km ^ j); }, [&](int j) { return t * 16 + (j ^ t); });
     Dft<16, true>::run(v);
     if (act) {
 #pragma unroll
@@ -1137,8 +1187,11 @@ template <int R> struct FusedGeom {
     static constexpr int G = g_for_R(R), GP = G + 1, SUB = G < kSub ? G : kSub, NSUB = G / SUB;
     static constexpr int NE = R * G, NSC = R * NSUB, ITSC = (NSC + kNT - 1) / kNT, ITE = (NE + kNT - 1) / kNT;
     // LDS: R x G elements with row stride GP (the padding spreads the sub-chunk reads over the
-    // banks), the column twiddles, one carry per sub-chunk, a 64-digit scratch for the fold
-    static constexpr size_t lds_bytes() { return (size_t)16 * (R * GP + R) + (size_t)8 * NSC + 4 * 64; }
+    // banks), the column twiddles, one carry per sub-chunk, the group's weight factors (inverse and
+    // forward: 2G column and R row values each), a 64-digit scratch for the fold, the digit codes
+    static constexpr size_t lds_bytes() {
+        return (size_t)16 * (R * GP + R) + (size_t)8 * NSC + (size_t)8 * (4 * G + 2 * R) + 4 * 64 + (size_t)R * G;
+    }
 };
 
 template <int R>
@@ -1156,8 +1209,23 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
     const uint32_t seq = s.seq + (uint32_t)step + 1;
     double2 *A = lds, *T = lds + R * GP;
     int64_t *SC = reinterpret_cast<int64_t *>(T + R);
-    int32_t *FD = reinterpret_cast<int32_t *>(SC + NSC);
-    for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+    double *WB = reinterpret_cast<double *>(SC + NSC);  // [2G] inverse column, [2G] forward column
+    double *WA = WB + 4 * G;                             // [R] inverse row (with k/(4H)), [R] forward row
+    int32_t *FD = reinterpret_cast<int32_t *>(WA + 2 * R);
+    uint8_t *CD = reinterpret_cast<uint8_t *>(FD + 64);  // [R][G] digit codes
+    for (int i = threadIdx.x; i < R; i += kNT) {
+        T[i] = sh.twR[i];
+        WA[i] = s.ta[R + i] * s.kscale;
+        WA[R + i] = s.ta[i];
+    }
+    for (int i = threadIdx.x; i < 2 * G; i += kNT) {
+        WB[i] = s.tb[2 * C + 2 * c0 + i];
+        WB[2 * G + i] = s.tb[2 * c0 + i];
+    }
+    {
+        const uint32_t *code4 = reinterpret_cast<const uint32_t *>(s.dcode + (size_t)g * R * G);
+        for (int i = threadIdx.x; i < R * G / 4; i += kNT) reinterpret_cast<uint32_t *>(CD)[i] = code4[i];
+    }
     const double2 *tg = sh.twCols + (size_t)g * NE;
 #pragma unroll
     for (int it = 0; it < ITE; it++) {
@@ -1169,8 +1237,6 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
     __syncthreads();
     fft_ip<R, 1, 0, G, GP, 1, true, true>(A, T);
     const int b0 = s.b0;
-    const uint8_t *code = s.dcode + (size_t)g * R * G;
-    const double *tbi = s.tb + 2 * C + 2 * c0;
     double lmax = 0;
     int32_t d[ITSC][2 * SUB];
     uint32_t bg[ITSC];  // bit i: digit i of the sub-chunk has b0+1 bits
@@ -1179,16 +1245,16 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
         const int w = it * kNT + (int)threadIdx.x;
         if (NSC % kNT != 0 && w >= NSC) continue;
         const int r = w / NSUB, q = w - r * NSUB;
-        const double tap = s.ta[R + r] * s.kscale;
+        const double tap = WA[r];
         double cy = 0;
         bg[it] = 0;
 #pragma unroll
         for (int l = 0; l < SUB; l++) {
             const int col = q * SUB + l;
             const double2 v = A[r * GP + col];
-            const int cd = code[r * G + col];
-            const double x0 = ldexp(v.x * (tap * tbi[2 * col]), -(cd & 3));
-            const double x1 = ldexp(v.y * (tap * tbi[2 * col + 1]), -((cd >> 3) & 3));
+            const int cd = CD[r * G + col];
+            const double x0 = ldexp(v.x * (tap * WB[2 * col]), -(cd & 3));
+            const double x1 = ldexp(v.y * (tap * WB[2 * col + 1]), -((cd >> 3) & 3));
             const double r0 = rint(x0), r1 = rint(x1);
             lmax = fmax(lmax, fmax(fabs(x0 - r0), fabs(x1 - r1)));
             const int b0f = (cd >> 2) & 1, b1f = (cd >> 5) & 1;
@@ -1214,7 +1280,6 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
     __syncthreads();
     // absorb the left neighbour's carry (within the work-group, or the previous chunk's through its
     // flag); the last digit keeps the rest. Then weight into A for the forward column FFT.
-    const double *tbf = s.tb + 2 * c0;
 #pragma unroll
     for (int it = 0; it < ITSC; it++) {
         const int w = it * kNT + (int)threadIdx.x;
@@ -1236,15 +1301,23 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
             }
             cy = bal((int64_t)d[it][i] + cy, b0 + (int)((bg[it] >> i) & 1), d[it][i]);
         }
-        const double tam = s.ta[r];
+        const double tam = WA[R + r];
 #pragma unroll
         for (int l = 0; l < SUB; l++) {
             const int col = q * SUB + l;
-            const int cd = code[r * G + col];
-            const double w0 = ldexp(tam * tbf[2 * col], cd & 3);
-            const double w1 = ldexp(tam * tbf[2 * col + 1], (cd >> 3) & 3);
+            const int cd = CD[r * G + col];
+            const double w0 = ldexp(tam * WB[2 * G + 2 * col], cd & 3);
+            const double w1 = ldexp(tam * WB[2 * G + 2 * col + 1], (cd >> 3) & 3);
             A[r * GP + col] = make_double2((double)d[it][2 * l] * w0, (double)d[it][2 * l + 1] * w1);
         }
+    }
+    // the four-step twiddles of the store, in flight during the forward FFT
+    double2 tws[ITE];
+#pragma unroll
+    for (int it = 0; it < ITE; it++) {
+        const int i = it * kNT + (int)threadIdx.x;
+        if (NE % kNT != 0 && i >= NE) continue;
+        tws[it] = gld(tg + i);
     }
     __syncthreads();
     fft_ip<R, 1, 0, G, GP, 1, true, false>(A, T);
@@ -1253,7 +1326,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
         const int i = it * kNT + (int)threadIdx.x;
         if (NE % kNT != 0 && i >= NE) continue;
         const int kr = i / G, l = i - kr * G;
-        gst(s.x + (size_t)kr * C + c0 + l, cmul(A[kr * GP + l], gld(tg + i)));
+        gst(s.x + (size_t)kr * C + c0 + l, cmul(A[kr * GP + l], tws[it]));
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
     if ((threadIdx.x & 63) == 0 && lmax > 0)
