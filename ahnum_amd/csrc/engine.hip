@@ -569,7 +569,7 @@ namespace kite {
 struct BatchSlot {
     kite_ctx *ctx = nullptr;
     std::vector<kite_num *> nums;
-    uint32_t seq = 0;  // fused column launches so far
+    int32_t xadd = 0;  // added to x after each squaring of this slot (LLR: -2)
 };
 
 }  // namespace kite
@@ -745,7 +745,8 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
         d.maxerr = b->d_maxerr + s;
         const bool on = xi[s] >= 0 && (size_t)xi[s] < b->slots[s].nums.size() && count[s] > 0;
         d.count = on ? count[s] : 0;
-        d.seq = b->slots[s].seq;
+        d.seq = ctx->eng->seq;  // one counter per engine: its own squarings use the same flags
+        d.xadd = yi ? 0 : b->slots[s].xadd;  // squarings only
         d.x = on ? (double2 *)b->slots[s].nums[xi[s]]->dev : nullptr;
         d.y_src = (on && yi && yi[s] >= 0) ? (const double2 *)b->slots[s].nums[yi[s]]->dev : nullptr;
     }
@@ -872,7 +873,7 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
     if (batch_upload(b, which, nullptr, count)) return -1;
     b->ev_stat.clear();
     for (size_t s = 0; s < b->slots.size(); s++)
-        if (b->slots[s].ctx) b->slots[s].seq += b->h_slots[s].count;  // the uploaded table holds the old value
+        if (b->slots[s].ctx) b->slots[s].ctx->eng->seq += b->h_slots[s].count;  // the table holds the old value
     for (uint32_t st = 0; st < maxc; st++) {
         const bool timed = b->timing && st % (uint32_t)b->timing == 0;
         batch_launch(b, 0, kModeSquare, (int)st, timed);
@@ -946,6 +947,12 @@ extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
                            (int)cap);
         HIP_OK(hipStreamSynchronize(b->stream));
     }
+    return 0;
+}
+
+extern "C" int kite_batch_set_xadd(kite_batch *b, int slot, int32_t xadd) {
+    if (slot < 0 || (size_t)slot >= b->slots.size() || !b->slots[slot].ctx) return -1;
+    b->slots[slot].xadd = xadd;
     return 0;
 }
 

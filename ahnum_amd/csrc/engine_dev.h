@@ -86,6 +86,7 @@ struct SlotDev {  // one number being operated on (one test in a batch)
     unsigned long long *maxerr; // max round-off (bits of a non-negative double)
     unsigned long long *flags;  // fused column pass: per (g, r) chunk, seq << 40 | carry (40 bits)
     uint32_t seq;               // fused column launches of this slot so far (step s uses seq + s + 1)
+    int32_t xadd;               // square mode: x -> x^2 + xadd (V += xadd 2^n, added in the top fold)
     double kscale;              // k / (4H)
     double rscale;              // 1 / R
     uint64_t k;

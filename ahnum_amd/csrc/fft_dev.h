@@ -341,7 +341,8 @@ __device__ __forceinline__ i128 floordiv_small(i128 h, uint32_t k) {
 // H = q*k + r  ->  q into digit 0, r*2^m rebalanced into the top digits (src/ctx.c:345-375).
 __device__ __forceinline__ int64_t fold_top_dev(const SlotDev &s, int32_t *d, int64_t carry) {
     const int t0 = s.top_i0, nt = s.top_n, m = s.top_m;
-    i128 Hh = (i128)carry << s.top_shift;
+    // the stored value is V = x 2^n: x + xadd is V + xadd 2^n, i.e. xadd 2^m above the top region's base
+    i128 Hh = ((i128)carry << s.top_shift) + ((i128)s.xadd << m);
     int pos = 0;
     for (int i = 0; i < nt; i++) {
         Hh += (i128)d[t0 + i] << pos;

@@ -81,6 +81,10 @@ void engine_reset_maxerr(kite_ctx *ctx);
 void engine_sync(kite_ctx *ctx);
 const char *engine_last_error();
 
+// ---- LLR helpers (llr.cpp) -------------------------------------------------------------------
+// x = V_k(P, -1) by the Lucas ladder (cP holds P, c2 holds 2; v1, t are scratch).
+void llr_lucas_vk(kite_ctx *ctx, uint64_t k, kite_num *x, kite_num *v1, kite_num *cP, kite_num *c2, kite_num *t);
+
 }  // namespace kite
 
 #endif

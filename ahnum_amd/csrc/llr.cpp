@@ -126,35 +126,6 @@ LlrHdr make_hdr(uint64_t k, uint64_t n, uint32_t P, uint64_t iter, size_t nl, ui
     return h;
 }
 
-// (V_m, V_{m+1}) from m = 0 over the bits of k, Q = -1 (src/llr.c:150-184)
-void lucas_vk(kite_ctx *ctx, uint64_t k, kite_num *x, kite_num *v1, kite_num *cP, kite_num *c2, kite_num *t) {
-    kite_set_u64(ctx, x, 2);
-    kite_copy(ctx, v1, cP);
-    uint64_t m = 0;
-    int top = 63;
-    while (top > 0 && !((k >> top) & 1)) top--;
-    for (int b = top; b >= 0; b--) {
-        const bool odd = m & 1;
-        kite_copy(ctx, t, x);
-        kite_mul(ctx, t, v1);  // V_{2m+1} = V_m V_{m+1} - P Q^m
-        if (odd) kite_add(ctx, t, cP);
-        else kite_sub(ctx, t, cP);
-        if ((k >> b) & 1) {
-            kite_square(ctx, v1);  // V_{2m+2} = V_{m+1}^2 - 2 Q^{m+1}
-            if (odd) kite_sub(ctx, v1, c2);
-            else kite_add(ctx, v1, c2);
-            kite_copy(ctx, x, t);
-            m = 2 * m + 1;
-        } else {
-            kite_square(ctx, x);  // V_{2m} = V_m^2 - 2 Q^m
-            if (odd) kite_add(ctx, x, c2);
-            else kite_sub(ctx, x, c2);
-            kite_copy(ctx, v1, t);
-            m = 2 * m;
-        }
-    }
-}
-
 bool all_zero(const uint64_t *l, size_t nl) {
     for (size_t i = 0; i < nl; i++)
         if (l[i]) return false;
@@ -196,7 +167,7 @@ int llr_run(uint64_t k, uint64_t n, uint32_t P, const kite_llr_opts *o, uint32_t
             have_state = true;
         } else {
             kite_reset_roundoff(ctx);
-            lucas_vk(ctx, k, x, v1, cP, c2, t);
+            llr_lucas_vk(ctx, k, x, v1, cP, c2, t);
             const double e = kite_max_roundoff(ctx);
             if (e > res->maxerr) res->maxerr = e;
             if (e > maxround || (roundoff_due && o->inject_roundoff_ladder)) {
@@ -263,6 +234,39 @@ int llr_run(uint64_t k, uint64_t n, uint32_t P, const kite_llr_opts *o, uint32_t
 }
 
 }  // namespace
+
+namespace kite {
+
+// V_k(P, -1) by the ladder (V_m, V_{m+1}) over the bits of k (src/llr.c:150-184).
+void llr_lucas_vk(kite_ctx *ctx, uint64_t k, kite_num *x, kite_num *v1, kite_num *cP, kite_num *c2, kite_num *t) {
+    kite_set_u64(ctx, x, 2);
+    kite_copy(ctx, v1, cP);
+    uint64_t m = 0;
+    int top = 63;
+    while (top > 0 && !((k >> top) & 1)) top--;
+    for (int b = top; b >= 0; b--) {
+        const bool odd = m & 1;
+        kite_copy(ctx, t, x);
+        kite_mul(ctx, t, v1);  // V_{2m+1} = V_m V_{m+1} - P Q^m
+        if (odd) kite_add(ctx, t, cP);
+        else kite_sub(ctx, t, cP);
+        if ((k >> b) & 1) {
+            kite_square(ctx, v1);  // V_{2m+2} = V_{m+1}^2 - 2 Q^{m+1}
+            if (odd) kite_sub(ctx, v1, c2);
+            else kite_add(ctx, v1, c2);
+            kite_copy(ctx, x, t);
+            m = 2 * m + 1;
+        } else {
+            kite_square(ctx, x);  // V_{2m} = V_m^2 - 2 Q^m
+            if (odd) kite_add(ctx, x, c2);
+            else kite_sub(ctx, x, c2);
+            kite_copy(ctx, v1, t);
+            m = 2 * m;
+        }
+    }
+}
+
+}  // namespace kite
 
 extern "C" int kite_llr_choose_P(uint64_t k, uint64_t n, uint32_t *P, uint64_t *factor) {
     uint32_t p = 1;

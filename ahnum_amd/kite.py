@@ -335,6 +335,80 @@ SIGNATURES.update({
 })
 
 
+JOB_PRP, JOB_LLR, JOB_PENDING = 0, 1, 1
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("k", c_u64), ("n", c_u64), ("kind", c_u32), ("base", c_u32), ("fft_len", c_u32),
+                ("arrival", c_dbl)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("is_prime", ctypes.c_int),
+        ("res64", c_u64),
+        ("factor", c_u64),
+        ("P", c_u32),
+        ("fft_len", c_u32),
+        ("maxerr", c_dbl),
+        ("gerbicz_errors", c_u32),
+        ("roundoff_retries", c_u32),
+        ("confirm_runs", c_u32),
+        ("start", c_dbl),
+        ("finish", c_dbl),
+    ]
+
+
+JOB_DONE_FN = ctypes.CFUNCTYPE(None, c_p, ctypes.c_size_t, ctypes.POINTER(JobResult))
+
+
+class RunOpts(ctypes.Structure):
+    _fields_ = [
+        ("slots", c_u32),
+        ("hbm_budget", c_u64),
+        ("block", c_u32),
+        ("gerbicz_blocks", c_u32),
+        ("max_roundoff", c_dbl),
+        ("horizon", c_dbl),
+        ("inject_fault_job", c_u64),
+        ("inject_fault_at", c_u64),
+        ("on_done", JOB_DONE_FN),
+        ("arg", c_p),
+    ]
+
+
+SIGNATURES.update({
+    "kite_batch_set_xadd": (ctypes.c_int, [c_p, ctypes.c_int, ctypes.c_int32]),
+    "kite_job_bytes": (c_u64, [c_u32, c_u32]),
+    "kite_batch_run": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(RunOpts),
+                                      ctypes.POINTER(JobResult)]),
+})
+
+
+def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fault_job=0, fault_at=0,
+              max_roundoff=0.0, on_done=None):
+    """Runs jobs [(kind, k, n, arrival), ...] (kind JOB_PRP or JOB_LLR; Fermat base 3) through
+    kite_batch_run. Returns (rc, [JobResult])."""
+    arr = (Job * len(jobs))(*[Job(k, n, kind, 3, 0, arrival) for kind, k, n, arrival in jobs])
+    o = RunOpts()
+    o.slots = slots
+    o.hbm_budget = hbm_budget
+    o.block = block
+    o.gerbicz_blocks = blocks
+    o.max_roundoff = max_roundoff
+    o.horizon = horizon
+    o.inject_fault_job = fault_job
+    o.inject_fault_at = fault_at
+    keep = None
+    if on_done is not None:
+        keep = JOB_DONE_FN(lambda a, i, r: on_done(i, r.contents))
+        o.on_done = keep
+    res = (JobResult * len(jobs))()
+    rc = lib().kite_batch_run(arr, len(jobs), ctypes.byref(o), res)
+    return rc, [res[i] for i in range(len(jobs))]
+
+
 def batch_prp(tests, slots=0, block=1000, blocks=0, fault_test=0, fault_at=0, on_done=None):
     """Runs Fermat base-3 tests [(k, n), ...] through the batched engine. Returns [(status, PrpResult)]."""
     arr = (BatchTest * len(tests))(*[BatchTest(k, n, 3, 0) for k, n in tests])

@@ -3,8 +3,9 @@
 // These entry points have no counterpart in the reference, which runs one test per CPU core
 // (tools/kite.c:54-105); they exist for callers that run many independent tests on one GPU. A
 // batch holds number slots for independent moduli N = k*2^n - 1 of any lengths; each call below
-// advances every slot with one launch per pass of the squaring (rows, column inverse + carry,
-// column forward), so the GPU works on all tests at once.
+// advances every slot with shared launches per pass of the squaring (rows, then the column pass:
+// one fused kernel for squarings at compile-time lengths, else column inverse + carry and column
+// forward), so the GPU works on all tests at once.
 #ifndef KITE_BATCH_H
 #define KITE_BATCH_H
 
@@ -30,8 +31,11 @@ int kite_batch_capacity(const kite_batch *b);  // highest slot id + 1
 // Call kite_batch_sync before converting a residue the batch has just worked on.
 kite_ctx *kite_batch_ctx(kite_batch *b, int slot);
 kite_num *kite_batch_num(kite_batch *b, int slot, uint32_t i);
-// For each slot s < capacity with which[s] >= 0: residue which[s] = residue^(2^count[s]).
+// For each slot s < capacity with which[s] >= 0: residue which[s] squared count[s] times
+// (x -> x^2 + xadd each time, see kite_batch_set_xadd).
 int kite_batch_square(kite_batch *b, const int32_t *which, const uint32_t *count);
+// Squarings of this slot become x -> x^2 + xadd (|xadd| < 2^20; 0 by default, -2 for LLR steps).
+int kite_batch_set_xadd(kite_batch *b, int slot, int32_t xadd);
 // For each slot s with dst[s] >= 0: residue dst[s] = residue dst[s] * residue src[s].
 int kite_batch_mul(kite_batch *b, const int32_t *dst, const int32_t *src);
 // For each slot s with dst[s] >= 0: residue dst[s] = residue src[s].
@@ -76,6 +80,55 @@ typedef struct {
 // returns 0 if every test succeeded, else the first failing status.
 int kite_batch_prp(const kite_batch_test *tests, size_t ntests, const kite_batch_prp_opts *o,
                    kite_prp_result *res, int *status);
+
+// ---- mixed Fermat / LLR jobs with arrival times (continuous batching, serving) ---------------
+
+enum { KITE_JOB_PRP = 0, KITE_JOB_LLR = 1 };
+enum { KITE_JOB_PENDING = 1 };  // status of a job the horizon stopped before it finished
+
+typedef struct {
+    uint64_t k, n;
+    uint32_t kind;      // KITE_JOB_PRP (Fermat, Gerbicz-checked) or KITE_JOB_LLR
+    uint32_t base;      // Fermat base (0 = 3)
+    uint32_t fft_len;   // 0 = automatic
+    double arrival;     // seconds after the start of kite_batch_run before which it may not start
+} kite_job;
+
+typedef struct {
+    int status;         // 0 done; kite_prp / kite_llr negative codes; KITE_JOB_PENDING
+    int is_prime;       // Fermat: probable prime; LLR: prime
+    uint64_t res64;     // Fermat: low 64 bits of base^(N-1); LLR: of the final V (0 if prime)
+    uint64_t factor;    // LLR: a factor found while choosing P (the test is then not run)
+    uint32_t P;         // LLR: the V-sequence parameter
+    uint32_t fft_len;   // length of the final run
+    double maxerr;
+    uint32_t gerbicz_errors, roundoff_retries, confirm_runs;
+    double start, finish;  // seconds after the start of kite_batch_run (finish: result known)
+} kite_job_result;
+
+typedef struct {
+    uint32_t slots;          // jobs in flight at most (0 = no limit)
+    uint64_t hbm_budget;     // device bytes for the jobs in flight at most (0 = no limit)
+    uint32_t block;          // squarings per round, the Fermat Gerbicz block B (0 = 1000)
+    uint32_t gerbicz_blocks; // Fermat: blocks between checks (0 = B/4)
+    double max_roundoff;     // 0 = 0.40
+    double horizon;          // seconds; jobs unfinished by then end KITE_JOB_PENDING (0 = none)
+    uint64_t inject_fault_job;  // testing: with inject_fault_at, corrupt job (index) at iteration
+    uint64_t inject_fault_at;   // (LLR: in its first run, so the vote has to outvote it)
+    void (*on_done)(void *arg, size_t index, const kite_job_result *r);
+    void *arg;
+} kite_run_opts;
+
+// Device bytes a job of this kind holds at length L while in flight (the admission estimate).
+uint64_t kite_job_bytes(uint32_t kind, uint32_t fft_len);
+
+// Runs the jobs through one batch: each job starts once its arrival time has passed and the slot
+// and HBM limits allow (first come, first served); every round advances all jobs in flight by up
+// to `block` squarings with shared launches, then checks each (Fermat: Gerbicz every M blocks with
+// rollback, doubled tail; LLR: round-off after every block with a verified copy, confirmation
+// runs at the next FFT length and a vote as kite_llr). A finished job's slot is refilled at
+// once. Returns 0 if every job finished with status 0, else the first other status.
+int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run_opts *o, kite_job_result *res);
 
 #ifdef __cplusplus
 }

@@ -126,3 +126,77 @@ def test_batch_subset_vs_gmp(golden):
     assert rc == 0
     for (k, n, base, want), (st, r) in zip(rows, out):
         assert st == 0 and "%016X" % r.res64 == want, (k, n)
+
+
+def test_batch_llr_matches_prst(golden):
+    """LLR jobs in one batch (the squarings subtract 2 inside the column kernels): primes and the
+    composite RES64s the reference checked against PRST 14.0.1888."""
+    rows = golden("llr_expect.csv")
+    jobs = [(kite.JOB_LLR, int(k), int(n), 0.0) for k, n, _ in rows]
+    rc, out = kite.batch_run(jobs, block=1000)
+    assert rc == 0
+    for (k, n, want), r in zip(rows, out):
+        say("llr", k, n, r.status, "%016X" % r.res64, r.confirm_runs, r.fft_len)
+        assert r.status == 0 and r.maxerr < 0.4
+        if want == "prime":
+            assert r.is_prime and not r.factor
+        else:
+            assert not r.is_prime and "%016X" % r.res64 == want
+            assert r.confirm_runs == 1  # a composite is confirmed once at the next length
+
+
+def test_batch_mixed_fermat_and_llr(golden):
+    """Fermat and LLR jobs sharing launches; each result equals the known one."""
+    jobs = [(kite.JOB_PRP, 9357, 20000, 0.0), (kite.JOB_LLR, 9357, 20000, 0.0), (kite.JOB_PRP, 8191, 25000, 0.0),
+            (kite.JOB_LLR, 3, 26459, 0.0), (kite.JOB_PRP, 3, 41628, 0.0), (kite.JOB_LLR, 33, 99999, 0.0)]
+    rc, out = kite.batch_run(jobs, block=1000)
+    assert rc == 0
+    assert "%016X" % out[0].res64 == "79291CD2F47CE48C" and not out[0].is_prime
+    assert "%016X" % out[1].res64 == "DC52061161A0DE1C" and not out[1].is_prime
+    assert "%016X" % out[2].res64 == "D45538E6F07266C6"
+    assert out[3].is_prime
+    assert out[4].is_prime and out[4].res64 == 1
+    assert "%016X" % out[5].res64 == "3FAF838186E8A836"
+
+
+def test_batch_llr_fault_outvoted():
+    """A corrupted first run of a prime's LLR job is outvoted by runs 1 and 2 (as kite_llr,
+    tests/test_llr.c:69-74), while a neighbouring job is unaffected."""
+    jobs = [(kite.JOB_LLR, 3, 41628, 0.0), (kite.JOB_LLR, 9357, 25000, 0.0)]
+    rc, out = kite.batch_run(jobs, block=1000, fault_job=0, fault_at=20000)
+    assert rc == 0
+    assert out[0].is_prime and out[0].confirm_runs == 2
+    assert "%016X" % out[1].res64 == "54A27E03D75F2447" and out[1].confirm_runs == 1
+
+
+def test_serving_arrivals_slots_and_horizon():
+    """Jobs start no earlier than they arrive, at most `slots` at a time, first come first served;
+    a job the horizon cuts off reports KITE_JOB_PENDING."""
+    jobs = [(kite.JOB_PRP, 9357, 20000, 0.0), (kite.JOB_LLR, 3, 26459, 0.3), (kite.JOB_PRP, 8191, 25000, 0.6),
+            (kite.JOB_LLR, 9357, 20000, 0.9)]
+    rc, out = kite.batch_run(jobs, slots=2, block=500)
+    assert rc == 0
+    for (kind, k, n, arr), r in zip(jobs, out):
+        say(kind, k, n, "arrival", arr, "start %.3f finish %.3f" % (r.start, r.finish))
+        assert r.status == 0 and r.start >= arr and r.finish > r.start
+    assert "%016X" % out[0].res64 == "79291CD2F47CE48C"
+    assert out[1].is_prime
+    assert "%016X" % out[2].res64 == "D45538E6F07266C6"
+    assert "%016X" % out[3].res64 == "DC52061161A0DE1C"
+    # a long job and a short horizon: the short one finishes, the long one is pending
+    rc, out = kite.batch_run([(kite.JOB_PRP, 9357, 20000, 0.0), (kite.JOB_PRP, 8045, 659321, 0.0)], horizon=2.0)
+    assert out[0].status == 0 and "%016X" % out[0].res64 == "79291CD2F47CE48C"
+    assert out[1].status == kite.JOB_PENDING and rc == kite.JOB_PENDING
+
+
+def test_hbm_budget_limits_jobs_in_flight():
+    """With an HBM budget for two jobs, never more than two run at once; results unchanged."""
+    L = kite.lib().kite_choose_fft_len(9357, 20000)
+    budget = 2 * kite.lib().kite_job_bytes(kite.JOB_PRP, L) + 1
+    jobs = [(kite.JOB_PRP, 9357, 20000, 0.0)] * 4
+    rc, out = kite.batch_run(jobs, hbm_budget=budget, block=1000)
+    assert rc == 0
+    spans = sorted((r.start, r.finish) for r in out)
+    for t, _ in spans:
+        assert sum(1 for s, f in spans if s <= t < f) <= 2
+    assert all("%016X" % r.res64 == "79291CD2F47CE48C" for r in out)

@@ -96,6 +96,26 @@ def test_arith_vs_exact_ints(k, n, L):
             v.free()
 
 
+@pytest.mark.parametrize("k,n,m", [
+    (883, 4046207, 4), (109, 5665875, 3), (497, 5902972, 3),  # the 3M-6M configuration (L 327680-458752)
+    (91, 24767982, 2),  # record size, L 1835008
+])
+def test_large_lengths_vs_exact_ints(k, n, m):
+    """Squarings at the large FFT lengths the selector picks for n = 3M..25M, checked against exact
+    integer squaring mod N."""
+    L = kite.lib().kite_choose_fft_len(k, n)
+    say("large", k, n, "L", L, "squarings", m)
+    r = random.Random(k + n)
+    with kite.Ctx(k, n) as c:
+        assert c.L == L
+        a = r.randrange(c.N)
+        x = c.num(a)
+        x.square(m)
+        assert x.get() == sqr_n(a, m, k, n)
+        assert c.maxerr() < 0.4
+        x.free()
+
+
 @pytest.mark.parametrize("order", [0, 1, 2, 3])
 def test_shape_variants_agree(order):
     """Every shape variant (R x C split) gives the same value (the autotuner may pick any)."""
