@@ -595,6 +595,12 @@ struct kite_batch {
     std::vector<Group> grows, gcols;
     unsigned long long *d_maxerr = nullptr;
     bool dirty = true;
+    uint64_t gen = 0;  // bumped whenever the job arrays or groups are rebuilt
+    // square calls replayed from a captured graph: valid for gsteps steps of generation ggen
+    bool graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t gsteps = 0;
+    uint64_t ggen = 0;
     int timing = 0;  // 0: off; N: every N-th step of a square call is timed
     // timing: one event pair per launch of the last timed call, and what each pair measured
     std::vector<hipEvent_t> ev;
@@ -727,6 +733,7 @@ static int batch_refresh(kite_batch *b) {
     b->nrows = (int)rows.size();
     b->ncols = (int)cols.size();
     b->dirty = false;
+    b->gen++;
     return 0;
 }
 
@@ -803,6 +810,7 @@ extern "C" void kite_batch_free(kite_batch *b) {
     hipStreamSynchronize(b->stream);
     for (size_t s = 0; s < b->slots.size(); s++) kite_batch_remove(b, (int)s);
     for (hipEvent_t e : b->ev) hipEventDestroy(e);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->d_slots) hipFree(b->d_slots);
     if (b->h_slots) hipHostFree(b->h_slots);
     if (b->d_rows) hipFree(b->d_rows);
@@ -874,11 +882,37 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
     b->ev_stat.clear();
     for (size_t s = 0; s < b->slots.size(); s++)
         if (b->slots[s].ctx) b->slots[s].ctx->eng->seq += b->h_slots[s].count;  // the table holds the old value
-    for (uint32_t st = 0; st < maxc; st++) {
-        const bool timed = b->timing && st % (uint32_t)b->timing == 0;
-        batch_launch(b, 0, kModeSquare, (int)st, timed);
-        batch_launch(b, 1, kModeSquare, (int)st, timed);
-        batch_launch(b, 2, kModeSquare, (int)st, timed);
+    auto launch_all = [&]() {
+        for (uint32_t st = 0; st < maxc; st++) {
+            const bool timed = b->timing && st % (uint32_t)b->timing == 0;
+            batch_launch(b, 0, kModeSquare, (int)st, timed);
+            batch_launch(b, 1, kModeSquare, (int)st, timed);
+            batch_launch(b, 2, kModeSquare, (int)st, timed);
+        }
+    };
+    // Kernels read the slot table at run time, so a captured sequence of maxc steps stays valid
+    // until the job arrays change (a new generation): the usual Gerbicz-block loop replays it.
+    bool replay = b->graphs && !b->timing && maxc > 1;
+    if (replay && (!b->gexec || b->gsteps != maxc || b->ggen != b->gen)) {
+        if (b->gexec) hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+        hipGraph_t g = nullptr;
+        replay = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (replay) {
+            launch_all();
+            replay = hipStreamEndCapture(b->stream, &g) == hipSuccess && g &&
+                     hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) == hipSuccess;
+            if (g) hipGraphDestroy(g);
+            if (!replay) b->gexec = nullptr;
+        }
+        (void)hipGetLastError();  // a failed capture falls back to direct launches
+        b->gsteps = maxc;
+        b->ggen = b->gen;
+    }
+    if (replay && b->gexec) {
+        if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return -1;
+    } else {
+        launch_all();
     }
     if (b->timing) {
         // per-pass and per-kernel totals of this call; algorithmic bytes per test and launch are
@@ -947,6 +981,11 @@ extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
                            (int)cap);
         HIP_OK(hipStreamSynchronize(b->stream));
     }
+    return 0;
+}
+
+extern "C" int kite_batch_set_graphs(kite_batch *b, int on) {
+    b->graphs = on != 0;
     return 0;
 }
 

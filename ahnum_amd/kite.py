@@ -380,6 +380,7 @@ class RunOpts(ctypes.Structure):
 
 SIGNATURES.update({
     "kite_batch_set_xadd": (ctypes.c_int, [c_p, ctypes.c_int, ctypes.c_int32]),
+    "kite_batch_set_graphs": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_job_bytes": (c_u64, [c_u32, c_u32]),
     "kite_batch_run": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(RunOpts),
                                       ctypes.POINTER(JobResult)]),
@@ -480,6 +481,10 @@ class Batch:
 
     def sync(self):
         self.lib.kite_batch_sync(self.p)
+
+    def graphs(self, on):
+        """Replay multi-step square calls from a captured HIP graph (the default) or launch directly."""
+        self.lib.kite_batch_set_graphs(self.p, int(bool(on)))
 
     def timing(self, every):
         """Time each launch of every `every`-th step of later square calls (0/False: off)."""

@@ -2,6 +2,7 @@
 """Benchmark of the MI355X squaring engine on the Fermat workload at n ~ 660K (FFT ~49152).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--cpu-secs S] [--no-cpu]
+  python bench.py --serve [--serve-count C] [--serve-rate R] [--serve-horizon S] ...
 
 A step is one squaring of every residue in the batch (B Fermat base-3 tests with k odd in
 [1001, 9999], n in [650000, 670000], seeded): the batched engine advances all B tests with one
@@ -9,8 +10,10 @@ launch per pass. The same number is also timed alone through the kite.h path (th
 one-test-at-a-time GPU path). tests/hour follows the reference's throughput formula
 (tools/kite.c:262): sum over tests of squarings/s * 3600 / n.
 
-Prints one JSON line. `roofline` is for the dominant pass kernel (the most device time), timed
-with HIP events on the batch stream around each launch of every 10th step of the timed region;
+The timed region runs square calls of 1000 steps (a Gerbicz block), each replayed from the HIP
+graph the batch captured for it. Prints one JSON line. `roofline` is for the dominant pass kernel
+(the most device time), timed right after in a second region of 1000 steps launched directly, with
+HIP events on the batch stream around each launch of every 10th step;
 algorithmic bytes per launch are 16 L per test for the row pass (read + write of the 8L-byte
 residue) and 8 L per test for each column pass (32 L per squaring in total). `traffic` is the
 kernel's HBM bytes per launch from profiles/hbm_traffic_single_large_batch.txt (rocprofv3
@@ -83,6 +86,51 @@ def load_traffic(kernel):
     return None
 
 
+def serve(a):
+    """Serving run: a finite Poisson stream of Fermat and LLR requests through kite_batch_run."""
+    from ahnum_amd import kite, workloads
+    stream = workloads.serve_stream(count=a.serve_count, rate=a.serve_rate, n_lo=a.serve_n_lo, n_hi=a.serve_n_hi)
+    jobs = [(kite.JOB_LLR if kind == "llr" else kite.JOB_PRP, k, n, t) for t, kind, k, n in stream]
+    t0 = time.perf_counter()
+    def progress(i, r):  # one line per completed request, on stderr
+        print(f"done {i} status {r.status} n {jobs[i][2]} latency {r.finish - jobs[i][3]:.2f}s", file=sys.stderr,
+              flush=True)
+
+    rc, out = kite.batch_run(jobs, hbm_budget=int(a.serve_hbm_gb * 2**30), block=1000, horizon=a.serve_horizon,
+                             on_done=progress)
+    wall = time.perf_counter() - t0
+    done = [(j, r) for j, r in zip(jobs, out) if r.status == 0]
+    lat = sorted(r.finish - j[3] for j, r in done)
+
+    def pct(p):
+        return round(lat[min(len(lat) - 1, int(p * len(lat)))], 3) if lat else None
+    horizon = min(wall, a.serve_horizon) if a.serve_horizon else wall
+    tph = len(done) * 3600.0 / horizon
+    out_line = {
+        "metric": "serving: completed tests/hour (Poisson arrivals, 50% Fermat + 50% LLR, n log-uniform)",
+        "value": round(tph, 1),
+        "unit": "tests/hour",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic request stream (seeded)",
+        "config": {"workload": "serve_stream", "requests": a.serve_count, "rate_per_s": a.serve_rate,
+                   "n_range": [a.serve_n_lo, a.serve_n_hi], "horizon_s": a.serve_horizon,
+                   "hbm_budget_gib": a.serve_hbm_gb, "block": 1000},
+        "completed": len(done),
+        "completed_fermat": sum(1 for j, _ in done if j[0] == kite.JOB_PRP),
+        "completed_llr": sum(1 for j, _ in done if j[0] == kite.JOB_LLR),
+        "backlog": sum(1 for r in out if r.status == kite.JOB_PENDING),
+        "failed": sum(1 for r in out if r.status not in (0, kite.JOB_PENDING)),
+        "latency_p50_s": pct(0.50),
+        "latency_p99_s": pct(0.99),
+        "max_completed_n": max((j[2] for j, _ in done), default=0),
+        "wall_s": round(wall, 2),
+        "maxerr": round(max((r.maxerr for _, r in done), default=0.0), 4),
+    }
+    print(json.dumps(out_line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,15 +141,28 @@ def main():
     ap.add_argument("--cpu-secs", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-every", type=int, default=10,
-                    help="bracket the launches of every N-th timed step with HIP events")
+                    help="bracket the launches of every N-th step of the kernel-timing region with HIP events")
+    ap.add_argument("--time-steps", type=int, default=1000, help="steps of the kernel-timing region")
+    ap.add_argument("--block", type=int, default=1000, help="squarings per square call (a Gerbicz block)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a fill, a copy and an add over this many MiB (calibrates the HBM "
                          "byte counters of a rocprofv3 --pmc run of this command)")
+    ap.add_argument("--serve", action="store_true", help="serving run instead of the batch benchmark")
+    ap.add_argument("--serve-count", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-n-lo", type=int, default=200000)
+    ap.add_argument("--serve-n-hi", type=int, default=5000000)
+    ap.add_argument("--serve-horizon", type=float, default=120.0)
+    ap.add_argument("--serve-hbm-gb", type=float, default=200.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (independent tests shard "
               f"across GPUs with no exchange; run one process per GPU)", file=sys.stderr)
         sys.exit(2)
+
+    if a.serve:
+        serve(a)
+        return
 
     from ahnum_amd import kite, workloads
 
@@ -140,17 +201,29 @@ def main():
         s = b.add(k, n)
         b.set(s, 0, workloads.random_residue_limbs(k * (1 << n) - 1, 100 + i))
         Ls.append(b.fft_len(s))
-    b.square(a.warmup)
+    # timed region 1 (the value): square calls of `block` steps, as the Gerbicz loop makes them,
+    # replayed from the captured HIP graph
+    block = min(a.block, a.steps)
+    b.square(block)  # captures the graph
+    b.square(max(a.warmup - block, 0))
     b.maxerr(reset=True)
-    b.timing(a.time_every)
     b.sync()
     t0 = time.perf_counter()
-    b.square(a.steps)
+    done = 0
+    while done < a.steps:
+        c = min(block, a.steps - done)
+        b.square(c)
+        done += c
     b.sync()
     wall = time.perf_counter() - t0
+    errs = b.maxerr()
+    # timed region 2 (the roofline): the same squarings launched directly, the launches of every
+    # `time_every`-th step bracketed by HIP events on the batch stream
+    b.timing(a.time_every)
+    b.square(a.time_steps)
+    b.sync()
     ms, launches = b.pass_times()
     kstats = b.kernel_times()
-    errs = b.maxerr()
     b.close()
 
     rate = a.steps / wall  # squarings/s per test
@@ -180,6 +253,7 @@ def main():
             "workload": "single_large batch: B Fermat base-3 tests, k odd in [1001,9999], n in [650000,670000]",
             "batch": a.batch,
             "fft_lengths": sorted(set(Ls)),
+            "launch": f"square calls of {block} steps replayed from a captured HIP graph",
             "seed": 2,
         },
         "roofline": {
