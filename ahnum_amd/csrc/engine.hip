@@ -591,8 +591,16 @@ struct kite_batch {
         kite::PassKernel kf = nullptr;  // columns: square-mode fused pass, if any
         int stat_f = -1;
         size_t lds_f = 0;
+        int part = 0;  // the stream the group's launches go to
     };
     std::vector<Group> grows, gcols;
+    // The slots are split into `nparts` parts with a stream each: the parts' launches run
+    // concurrently, so their kernels overlap in different phases (loads, compute, stores).
+    static constexpr int kMaxParts = 4;
+    int nparts = 2;
+    hipStream_t ps[kMaxParts] = {};  // ps[0] == stream
+    hipEvent_t evfork = nullptr, evtable = nullptr, evjoin[kMaxParts] = {};
+    std::vector<int> slot_part;
     unsigned long long *d_maxerr = nullptr;
     bool dirty = true;
     uint64_t gen = 0;  // bumped whenever the job arrays or groups are rebuilt
@@ -618,8 +626,14 @@ struct kite_batch {
 
 using namespace kite;
 
+static void batch_sync_all(kite_batch *b) {
+    for (int p = 0; p < kite_batch::kMaxParts; p++)
+        if (b->ps[p]) hipStreamSynchronize(b->ps[p]);
+}
+
 static int batch_refresh(kite_batch *b) {
     if (!b->dirty) return 0;
+    batch_sync_all(b);  // the tables below may be reallocated
     const size_t cap = b->slots.size();
     if (cap > b->slot_cap) {
         const size_t nc = std::max<size_t>(cap, 2 * b->slot_cap);
@@ -653,12 +667,20 @@ static int batch_refresh(kite_batch *b) {
         b->kstats.push_back(ks);
         return (int)b->kstats.size() - 1;
     };
-    auto build = [&](bool isrows) {
+    // parts: active slots dealt round-robin (by slot order) to the parts
+    b->slot_part.assign(cap, 0);
+    {
+        int i = 0;
+        for (int s : order) b->slot_part[s] = (i++) % b->nparts;
+    }
+    auto build = [&](bool isrows, int part) {
         std::vector<int2> &jobs = isrows ? rows : cols;
-        std::vector<kite_batch::Group> &grp = isrows ? b->grows : b->gcols;
+        std::vector<kite_batch::Group> grp;
         std::vector<int> gR;  // column length of each group
         std::vector<bool> done(cap, false);
-        std::vector<int> ord = order;
+        std::vector<int> ord;
+        for (int s : order)
+            if (b->slot_part[s] == part) ord.push_back(s);
         if (!isrows)  // column groups in ascending R, so that adjacent lengths sit side by side
             std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
                 return b->slots[x].ctx->eng->shape->ch.R < b->slots[y].ctx->eng->shape->ch.R;
@@ -687,10 +709,15 @@ static int batch_refresh(kite_batch *b) {
                 g.kf = E0->kcfused;
                 if (g.kf) g.stat_f = stat_of("k_cols_fused_r<" + std::to_string(E0->shape->ch.R) + ">", 1);
             }
+            g.part = part;
             grp.push_back(g);
             gR.push_back((int)E0->shape->ch.R);
         }
-        if (isrows) return;
+        std::vector<kite_batch::Group> &out = isrows ? b->grows : b->gcols;
+        if (isrows) {
+            out.insert(out.end(), grp.begin(), grp.end());
+            return;
+        }
         // adjacent compile-time column lengths share one launch per pass
         std::vector<kite_batch::Group> merged;
         for (size_t i = 0; i < grp.size(); i++) {
@@ -715,10 +742,12 @@ static int batch_refresh(kite_batch *b) {
                 merged.push_back(grp[i]);
             }
         }
-        grp = merged;
+        out.insert(out.end(), merged.begin(), merged.end());
     };
-    build(true);
-    build(false);
+    for (int part = 0; part < b->nparts; part++) {
+        build(true, part);
+        build(false, part);
+    }
     const size_t need = std::max(rows.size(), cols.size());
     if (need > b->jobs_cap) {
         if (b->d_rows) hipFree(b->d_rows);
@@ -740,7 +769,7 @@ static int batch_refresh(kite_batch *b) {
 // Fills the slot table: residue `x` (and operand `y`) per slot, step count per slot (0 = idle).
 static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, const uint32_t *count) {
     if (batch_refresh(b)) return -1;
-    HIP_OK(hipStreamSynchronize(b->stream));  // the previous table may still be read
+    batch_sync_all(b);  // the previous table may still be read
     for (size_t s = 0; s < b->slots.size(); s++) {
         SlotDev &d = b->h_slots[s];
         kite_ctx *ctx = b->slots[s].ctx;
@@ -759,6 +788,11 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
     }
     HIP_OK(hipMemcpyAsync(b->d_slots, b->h_slots, b->slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice,
                           b->stream));
+    // the other parts' launches read the table too: they wait for the copy
+    if (b->nparts > 1) {
+        HIP_OK(hipEventRecord(b->evtable, b->stream));
+        for (int p = 1; p < b->nparts; p++) HIP_OK(hipStreamWaitEvent(b->ps[p], b->evtable, 0));
+    }
     return 0;
 }
 
@@ -784,11 +818,11 @@ static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed
                 hipEventCreate(&b->ev[e + 1]);
             }
             b->ev_stat.push_back(fused ? g.stat_f : g.stat[pass == 2 ? 1 : 0]);
-            hipEventRecord(b->ev[e], b->stream);
-            launch_pass(fn, g.count, lds, b->stream, b->d_slots, none, jobs + g.off, mode, step, threads);
-            hipEventRecord(b->ev[e + 1], b->stream);
+            hipEventRecord(b->ev[e], b->ps[g.part]);
+            launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, none, jobs + g.off, mode, step, threads);
+            hipEventRecord(b->ev[e + 1], b->ps[g.part]);
         } else {
-            launch_pass(fn, g.count, lds, b->stream, b->d_slots, none, jobs + g.off, mode, step, threads);
+            launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, none, jobs + g.off, mode, step, threads);
         }
     }
 }
@@ -802,12 +836,18 @@ extern "C" kite_batch *kite_batch_new(void) {
         delete b;
         return nullptr;
     }
+    b->ps[0] = b->stream;
+    for (int p = 1; p < kite_batch::kMaxParts; p++)
+        if (hipStreamCreateWithFlags(&b->ps[p], hipStreamNonBlocking) != hipSuccess) b->ps[p] = nullptr;
+    hipEventCreateWithFlags(&b->evfork, hipEventDisableTiming);
+    hipEventCreateWithFlags(&b->evtable, hipEventDisableTiming);
+    for (int p = 0; p < kite_batch::kMaxParts; p++) hipEventCreateWithFlags(&b->evjoin[p], hipEventDisableTiming);
     return b;
 }
 
 extern "C" void kite_batch_free(kite_batch *b) {
     if (!b) return;
-    hipStreamSynchronize(b->stream);
+    batch_sync_all(b);
     for (size_t s = 0; s < b->slots.size(); s++) kite_batch_remove(b, (int)s);
     for (hipEvent_t e : b->ev) hipEventDestroy(e);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
@@ -816,7 +856,12 @@ extern "C" void kite_batch_free(kite_batch *b) {
     if (b->d_rows) hipFree(b->d_rows);
     if (b->d_cols) hipFree(b->d_cols);
     if (b->d_maxerr) hipFree(b->d_maxerr);
-    hipStreamDestroy(b->stream);
+    for (int p = 0; p < kite_batch::kMaxParts; p++) {
+        if (b->ps[p]) hipStreamDestroy(b->ps[p]);
+        if (b->evjoin[p]) hipEventDestroy(b->evjoin[p]);
+    }
+    if (b->evfork) hipEventDestroy(b->evfork);
+    if (b->evtable) hipEventDestroy(b->evtable);
     delete b;
 }
 
@@ -852,7 +897,7 @@ extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t ff
 
 extern "C" void kite_batch_remove(kite_batch *b, int slot) {
     if (slot < 0 || (size_t)slot >= b->slots.size() || !b->slots[slot].ctx) return;
-    hipStreamSynchronize(b->stream);
+    batch_sync_all(b);
     BatchSlot &bs = b->slots[slot];
     for (kite_num *x : bs.nums) kite_num_free(bs.ctx, x);
     kite_ctx_free(bs.ctx);
@@ -871,7 +916,21 @@ extern "C" kite_num *kite_batch_num(kite_batch *b, int slot, uint32_t i) {
     return b->slots[slot].nums[i];
 }
 
-extern "C" int kite_batch_sync(kite_batch *b) { return hipStreamSynchronize(b->stream) == hipSuccess ? 0 : -1; }
+extern "C" int kite_batch_sync(kite_batch *b) {
+    batch_sync_all(b);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int kite_batch_set_parts(kite_batch *b, int parts) {
+    int avail = 0;
+    while (avail < kite_batch::kMaxParts && b->ps[avail]) avail++;
+    parts = std::max(1, std::min(parts, avail));
+    if (parts != b->nparts) {
+        b->nparts = parts;
+        b->dirty = true;
+    }
+    return parts;
+}
 
 extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint32_t *count) {
     uint32_t maxc = 0;
@@ -899,7 +958,14 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
         hipGraph_t g = nullptr;
         replay = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (replay) {
+            // the other parts' streams join the capture (fork) and rejoin the first (join)
+            hipEventRecord(b->evfork, b->stream);
+            for (int p = 1; p < b->nparts; p++) hipStreamWaitEvent(b->ps[p], b->evfork, 0);
             launch_all();
+            for (int p = 1; p < b->nparts; p++) {
+                hipEventRecord(b->evjoin[p], b->ps[p]);
+                hipStreamWaitEvent(b->stream, b->evjoin[p], 0);
+            }
             replay = hipStreamEndCapture(b->stream, &g) == hipSuccess && g &&
                      hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) == hipSuccess;
             if (g) hipGraphDestroy(g);
@@ -910,14 +976,22 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
         b->ggen = b->gen;
     }
     if (replay && b->gexec) {
+        // the graph runs every part on the first stream: order it after the other streams' work,
+        // and their later work (copies) after it
+        for (int p = 1; p < b->nparts; p++) {
+            hipEventRecord(b->evjoin[p], b->ps[p]);
+            hipStreamWaitEvent(b->stream, b->evjoin[p], 0);
+        }
         if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return -1;
+        hipEventRecord(b->evfork, b->stream);
+        for (int p = 1; p < b->nparts; p++) hipStreamWaitEvent(b->ps[p], b->evfork, 0);
     } else {
         launch_all();
     }
     if (b->timing) {
         // per-pass and per-kernel totals of this call; algorithmic bytes per test and launch are
         // 16 L for the row pass (residue read + written) and 8 L for each column pass
-        hipStreamSynchronize(b->stream);
+        batch_sync_all(b);
         for (int p = 0; p < 3; p++) {
             b->ms[p] = 0;
             b->launches[p] = (maxc + b->timing - 1) / b->timing;
@@ -959,11 +1033,12 @@ extern "C" int kite_batch_mul(kite_batch *b, const int32_t *dst, const int32_t *
 }
 
 extern "C" int kite_batch_copy(kite_batch *b, const int32_t *dst, const int32_t *src) {
+    if (batch_refresh(b)) return -1;
     for (size_t s = 0; s < b->slots.size(); s++) {
         BatchSlot &bs = b->slots[s];
         if (!bs.ctx || dst[s] < 0 || dst[s] == src[s]) continue;
         hipMemcpyAsync(bs.nums[dst[s]]->dev, bs.nums[src[s]]->dev, (size_t)bs.ctx->eng->H * sizeof(double2),
-                       hipMemcpyDeviceToDevice, b->stream);
+                       hipMemcpyDeviceToDevice, b->ps[b->slot_part[s]]);  // ordered with the slot's launches
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -973,6 +1048,7 @@ extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
     const size_t cap = b->slots.size();
     if (!cap) return 0;
     std::vector<unsigned long long> v(cap);
+    batch_sync_all(b);
     HIP_OK(hipMemcpyAsync(v.data(), b->d_maxerr, cap * 8, hipMemcpyDeviceToHost, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     for (size_t s = 0; s < cap; s++) memcpy(&out[s], &v[s], 8);

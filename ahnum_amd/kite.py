@@ -381,6 +381,7 @@ class RunOpts(ctypes.Structure):
 SIGNATURES.update({
     "kite_batch_set_xadd": (ctypes.c_int, [c_p, ctypes.c_int, ctypes.c_int32]),
     "kite_batch_set_graphs": (ctypes.c_int, [c_p, ctypes.c_int]),
+    "kite_batch_set_parts": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_job_bytes": (c_u64, [c_u32, c_u32]),
     "kite_batch_run": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(RunOpts),
                                       ctypes.POINTER(JobResult)]),
@@ -481,6 +482,10 @@ class Batch:
 
     def sync(self):
         self.lib.kite_batch_sync(self.p)
+
+    def parts(self, n):
+        """Deal the slots to n concurrent streams (1..4); returns the count in use."""
+        return self.lib.kite_batch_set_parts(self.p, int(n))
 
     def graphs(self, on):
         """Replay multi-step square calls from a captured HIP graph (the default) or launch directly."""

@@ -144,6 +144,7 @@ def main():
                     help="bracket the launches of every N-th step of the kernel-timing region with HIP events")
     ap.add_argument("--time-steps", type=int, default=1000, help="steps of the kernel-timing region")
     ap.add_argument("--block", type=int, default=1000, help="squarings per square call (a Gerbicz block)")
+    ap.add_argument("--parts", type=int, default=2, help="concurrent streams the batch is dealt to")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a fill, a copy and an add over this many MiB (calibrates the HBM "
                          "byte counters of a rocprofv3 --pmc run of this command)")
@@ -196,6 +197,7 @@ def main():
 
     # ---- batched engine ----
     b = kite.Batch()
+    parts = b.parts(a.parts)
     Ls = []
     for i, (k, n) in enumerate(tests):
         s = b.add(k, n)
@@ -219,6 +221,7 @@ def main():
     errs = b.maxerr()
     # timed region 2 (the roofline): the same squarings launched directly, the launches of every
     # `time_every`-th step bracketed by HIP events on the batch stream
+    b.parts(1)  # kernels alone, so that each launch's duration is its own
     b.timing(a.time_every)
     b.square(a.time_steps)
     b.sync()
@@ -253,7 +256,7 @@ def main():
             "workload": "single_large batch: B Fermat base-3 tests, k odd in [1001,9999], n in [650000,670000]",
             "batch": a.batch,
             "fft_lengths": sorted(set(Ls)),
-            "launch": f"square calls of {block} steps replayed from a captured HIP graph",
+            "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams",
             "seed": 2,
         },
         "roofline": {

@@ -34,6 +34,9 @@ kite_num *kite_batch_num(kite_batch *b, int slot, uint32_t i);
 // For each slot s < capacity with which[s] >= 0: residue which[s] squared count[s] times
 // (x -> x^2 + xadd each time, see kite_batch_set_xadd).
 int kite_batch_square(kite_batch *b, const int32_t *which, const uint32_t *count);
+// The slots are dealt to `parts` streams (1..4, default 2) whose launches run concurrently, so
+// kernels of different parts overlap their memory and compute phases. Returns the count in use.
+int kite_batch_set_parts(kite_batch *b, int parts);
 // Square calls of more than one step are captured as a HIP graph and replayed while the batch's
 // slots are unchanged (on by default; the launches are the same either way).
 int kite_batch_set_graphs(kite_batch *b, int on);

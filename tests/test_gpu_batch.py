@@ -200,3 +200,49 @@ def test_hbm_budget_limits_jobs_in_flight():
     for t, _ in spans:
         assert sum(1 for s, f in spans if s <= t < f) <= 2
     assert all("%016X" % r.res64 == "79291CD2F47CE48C" for r in out)
+
+
+@pytest.mark.parametrize("parts,graphs", [(1, True), (2, False), (3, True), (4, True)])
+def test_parts_streams_ordering(parts, graphs):
+    """Slots dealt to several streams: single-step and multi-step squarings, multiplies (whose
+    launches read the slot table uploaded on the first stream) and a slot removed and re-added
+    between calls; every value exact."""
+    import ctypes
+    cases = [(9357, 20000), (8191, 25000), (3, 26459), (7, 4001), (4523, 67231)]
+    b = kite.Batch()
+    r = random.Random(parts)
+    try:
+        assert b.parts(parts) == parts
+        b.graphs(graphs)
+        vals = {}
+        for k, n in cases:
+            s = b.add(k, n, nnums=2)
+            x, y = r.randrange(k * (1 << n) - 1), r.randrange(k * (1 << n) - 1)
+            b.set(s, 0, x)
+            b.set(s, 1, y)
+            vals[s] = [k, n, x, y]
+        cap = b.lib.kite_batch_capacity(b.p)
+        dst = (ctypes.c_int32 * cap)(*[0] * cap)
+        src = (ctypes.c_int32 * cap)(*[1] * cap)
+        for rnd in range(3):
+            b.square(1)
+            b.square(7)
+            assert b.lib.kite_batch_mul(b.p, dst, src) == 0
+            for s, v in vals.items():
+                k, n = v[0], v[1]
+                v[2] = mod_riesel(sqr_n(v[2], 8, k, n) * v[3], k, n)
+            if rnd == 1:  # remove one slot and add a new test in its place
+                s0 = next(iter(vals))
+                b.lib.kite_batch_remove(b.p, s0)
+                del vals[s0]
+                k, n = 1203, 90001
+                s = b.add(k, n, nnums=2)
+                x, y = r.randrange(k * (1 << n) - 1), r.randrange(k * (1 << n) - 1)
+                b.set(s, 0, x)
+                b.set(s, 1, y)
+                vals[s] = [k, n, x, y]
+        for s, (k, n, x, y) in vals.items():
+            assert b.get(s, 0) == x, (parts, k, n)
+        assert max(b.maxerr().values()) < 0.4
+    finally:
+        b.close()
