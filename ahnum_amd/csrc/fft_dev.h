@@ -1265,20 +1265,30 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
             d[it][2 * l] = (int32_t)dd;
             cy = bal_e(r1 + cy, b0 + b1f, dd);
             d[it][2 * l + 1] = (int32_t)dd;
+            // the chain is sequential: reading further ahead only costs registers
+            if (l % 2 == 1) __builtin_amdgcn_sched_barrier(0);
         }
-        int64_t co = (int64_t)cy;
-        if (g == NG - 1 && w == NSC - 1) {  // the top fold on the number's last sub-chunk
+        const int64_t co = (int64_t)cy;
+        const bool folder = g == NG - 1 && w == NSC - 1;  // holds the number's last sub-chunk
+        if (folder) {
 #pragma unroll
             for (int i = 0; i < 2 * SUB; i++) FD[i] = d[it][i];
-            co = fold_top_dev(s, FD, co);
-#pragma unroll
-            for (int i = 0; i < 2 * SUB; i++) d[it][i] = FD[i];
         }
         SC[w] = co;
-        if (q == NSUB - 1)  // this row chunk's carry, for the next chunk's work-group
+        if (q == NSUB - 1 && !folder)  // this row chunk's carry, for the next chunk's work-group
             __hip_atomic_store(s.flags + g * R + r, pack_flag(seq, co), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
+    // the top fold, here where few registers are live (its 128-bit arithmetic inside the loop above
+    // made every thread spill); the last sub-chunk's carry goes nowhere inside the work-group
+    if (g == NG - 1 && (int)threadIdx.x == (NSC - 1) % kNT) {
+        constexpr int it = ITSC - 1;
+        const int64_t co = fold_top_dev(s, FD, SC[NSC - 1]);
+        SC[NSC - 1] = co;
+        __hip_atomic_store(s.flags + g * R + R - 1, pack_flag(seq, co), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int i = 0; i < 2 * SUB; i++) d[it][i] = FD[i];
+    }
     // absorb the left neighbour's carry (within the work-group, or the previous chunk's through its
     // flag); the last digit keeps the rest. Then weight into A for the forward column FFT.
 #pragma unroll
@@ -1312,22 +1322,31 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
             A[r * GP + col] = make_double2((double)d[it][2 * l] * w0, (double)d[it][2 * l + 1] * w1);
         }
     }
-    // the four-step twiddles of the store, in flight during the forward FFT
+    // the four-step twiddles of the store, in flight during the forward FFT (from a laundered base,
+    // so that the load phase's addresses are not kept live for it)
     double2 tws[ITE];
+    const double2 *tgs = tg;
+    asm volatile("" : "+v"(tgs));
 #pragma unroll
     for (int it = 0; it < ITE; it++) {
         const int i = it * kNT + (int)threadIdx.x;
         if (NE % kNT != 0 && i >= NE) continue;
-        tws[it] = gld(tg + i);
+        tws[it] = gld(tgs + i);
     }
     __syncthreads();
     fft_ip<R, 1, 0, G, GP, 1, true, false>(A, T);
+    // the store addresses are recomputed from laundered bases: otherwise the compiler keeps the
+    // load phase's 64-bit addresses live (spilled) through the whole kernel
+    double2 *xo = s.x + c0;
+    const double2 *Ao = A;
+    int Co = C;
+    asm volatile("" : "+v"(xo), "+v"(Co), "+v"(Ao));
 #pragma unroll
     for (int it = 0; it < ITE; it++) {
         const int i = it * kNT + (int)threadIdx.x;
         if (NE % kNT != 0 && i >= NE) continue;
         const int kr = i / G, l = i - kr * G;
-        gst(s.x + (size_t)kr * C + c0 + l, cmul(A[kr * GP + l], tws[it]));
+        gst(xo + kr * Co + l, cmul(Ao[kr * GP + l], tws[it]));
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
     if ((threadIdx.x & 63) == 0 && lmax > 0)
