@@ -689,6 +689,73 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
     }
 }
 
+// fft_ip for HOW column sequences (SEQFAST, SS = 1, no padding function) whose first stage may take
+// its inputs from ld(it, r, row, col) instead of the buffer (LDIN) and whose last stage may hand its
+// outputs to st(it, r, row, col, v) instead of the buffer (STOUT): a load or store phase fused into
+// the FFT saves a pass through LDS and its barriers. (it, r) name the thread's iteration and the
+// butterfly input/output, (row, col) the element.
+template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, class LD, class ST>
+__device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST &st) {
+    constexpr int NST = ct_nstages(N);
+    if constexpr (I < NST) {
+        constexpr int P = ct_radix(N, I), NB = N / P, TOTAL = NB * HOW;
+        constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
+        constexpr bool FROM_LD = LDIN && I == 0, TO_ST = STOUT && I == NST - 1;
+        double2 v[ITER][P];
+        int orow[ITER], seqs[ITER];
+#pragma unroll
+        for (int it = 0; it < ITER; it++) {
+            const int w = it * kNT + (int)threadIdx.x;
+            if (TOTAL % kNT == 0 || w < TOTAL) {
+                const int seq = w % HOW, j = w / HOW, jm = j % NS;
+#pragma unroll
+                for (int r = 0; r < P; r++) {
+                    if constexpr (FROM_LD) v[it][r] = ld(it, r, j + r * NB, seq);
+                    else v[it][r] = a[(j + r * NB) * IS + seq];
+                }
+                if (NS > 1 && jm) {
+#pragma unroll
+                    for (int r = 1; r < P; r++) {
+                        const double2 t = tw[jm * r * TS];
+                        v[it][r] = INV ? cmulc(v[it][r], t) : cmul(v[it][r], t);
+                    }
+                }
+                Dft<P, INV>::run(v[it]);
+                orow[it] = (j / NS) * NS * P + jm;
+                seqs[it] = seq;
+            }
+        }
+        if constexpr (TO_ST) {
+#pragma unroll
+            for (int it = 0; it < ITER; it++) {
+                const int w = it * kNT + (int)threadIdx.x;
+                if (TOTAL % kNT == 0 || w < TOTAL) {
+#pragma unroll
+                    for (int r = 0; r < P; r++) st(it, r, orow[it] + r * NS, seqs[it], v[it][r]);
+                }
+            }
+        } else {
+            if constexpr (!FROM_LD) __syncthreads();  // the stage is in place: reads before writes
+#pragma unroll
+            for (int it = 0; it < ITER; it++) {
+                const int w = it * kNT + (int)threadIdx.x;
+                if (TOTAL % kNT == 0 || w < TOTAL) {
+#pragma unroll
+                    for (int r = 0; r < P; r++) a[(orow[it] + r * NS) * IS + seqs[it]] = v[it][r];
+                }
+            }
+            __syncthreads();
+            fft_io<N, NS * P, I + 1, HOW, IS, INV, LDIN, STOUT>(a, tw, ld, st);
+        }
+    }
+}
+
+// The last stage's thread mapping of fft_io (for prefetching what its outputs need).
+template <int N, int HOW> struct LastStage {
+    static constexpr int P = ct_radix(N, ct_nstages(N) - 1), NB = N / P, TOTAL = NB * HOW;
+    static constexpr int ITER = (TOTAL + kNT - 1) / kNT;
+};
+
 // Rows: 2*PPW rows of C in LDS (row slot 2i+h = side h of local pair i), twiddles w_C after them.
 template <int C>
 __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
@@ -1228,15 +1295,14 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
         for (int i = threadIdx.x; i < R * G / 4; i += kNT) reinterpret_cast<uint32_t *>(CD)[i] = code4[i];
     }
     const double2 *tg = sh.twCols + (size_t)g * NE;
-#pragma unroll
-    for (int it = 0; it < ITE; it++) {
-        const int i = it * kNT + (int)threadIdx.x;
-        if (NE % kNT != 0 && i >= NE) continue;
-        const int kr = i / G, l = i - kr * G;
-        A[kr * GP + l] = cmulc(gld(s.x + (size_t)kr * C + c0 + l), gld(tg + i));
+    // inverse column FFT, its first stage reading the residue (times the conjugate four-step
+    // twiddle) straight from HBM; the tables above are published by its first barrier
+    {
+        const double2 *xin = s.x + c0;
+        auto ld = [&](int, int, int row, int col) { return cmulc(gld(xin + (size_t)row * C + col), gld(tg + row * G + col)); };
+        auto st = [](int, int, int, int, double2) {};
+        fft_io<R, 1, 0, G, GP, true, true, false>(A, T, ld, st);
     }
-    __syncthreads();
-    fft_ip<R, 1, 0, G, GP, 1, true, true>(A, T);
     const int b0 = s.b0;
     double lmax = 0;
     int32_t d[ITSC][2 * SUB];
@@ -1322,31 +1388,29 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
             A[r * GP + col] = make_double2((double)d[it][2 * l] * w0, (double)d[it][2 * l + 1] * w1);
         }
     }
-    // the four-step twiddles of the store, in flight during the forward FFT (from a laundered base,
-    // so that the load phase's addresses are not kept live for it)
-    double2 tws[ITE];
+    // forward column FFT, its last stage writing the residue (times the four-step twiddle) straight
+    // to HBM. The twiddles of the thread's last-stage outputs are fetched now, during the FFT; the
+    // bases are laundered so that the inverse side's 64-bit addresses are not kept live (spilled)
+    // through the whole kernel for reuse here.
+    using LS = LastStage<R, G>;
+    double2 tws[LS::ITER][LS::P];
     const double2 *tgs = tg;
-    asm volatile("" : "+v"(tgs));
+    double2 *xo = s.x + c0;
+    int Co = C;
+    asm volatile("" : "+v"(tgs), "+v"(xo), "+v"(Co));
 #pragma unroll
-    for (int it = 0; it < ITE; it++) {
-        const int i = it * kNT + (int)threadIdx.x;
-        if (NE % kNT != 0 && i >= NE) continue;
-        tws[it] = gld(tgs + i);
+    for (int it = 0; it < LS::ITER; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        if (LS::TOTAL % kNT == 0 || w < LS::TOTAL) {
+#pragma unroll
+            for (int r = 0; r < LS::P; r++) tws[it][r] = gld(tgs + (w / G + r * LS::NB) * G + w % G);
+        }
     }
     __syncthreads();
-    fft_ip<R, 1, 0, G, GP, 1, true, false>(A, T);
-    // the store addresses are recomputed from laundered bases: otherwise the compiler keeps the
-    // load phase's 64-bit addresses live (spilled) through the whole kernel
-    double2 *xo = s.x + c0;
-    const double2 *Ao = A;
-    int Co = C;
-    asm volatile("" : "+v"(xo), "+v"(Co), "+v"(Ao));
-#pragma unroll
-    for (int it = 0; it < ITE; it++) {
-        const int i = it * kNT + (int)threadIdx.x;
-        if (NE % kNT != 0 && i >= NE) continue;
-        const int kr = i / G, l = i - kr * G;
-        gst(xo + kr * Co + l, cmul(Ao[kr * GP + l], tws[it]));
+    {
+        auto ld = [](int, int, int, int) { return make_double2(0, 0); };
+        auto st = [&](int it, int r, int row, int col, double2 v) { gst(xo + row * Co + col, cmul(v, tws[it][r])); };
+        fft_io<R, 1, 0, G, GP, false, false, true>(A, T, ld, st);
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
     if ((threadIdx.x & 63) == 0 && lmax > 0)
