@@ -328,6 +328,7 @@ int fill_slot_fold(kite_ctx *ctx, const ShapeDev &sh, SlotDev *s) {
     const uint32_t sub = sh.G < kSub ? (uint32_t)sh.G : (uint32_t)kSub;
     const uint32_t L = ctx->L, chunk0 = L - 2 * sub;  // the fold works on the last sub-chunk
     s->k = ctx->k;
+    s->kinv = ~0ull / ctx->k;
     s->b0 = (int)ctx->b0;
     s->top_n = (int)(L - ctx->top_t);
     if (s->top_n > (int)(2 * sub) || s->top_n > kMaxTop) return -1;

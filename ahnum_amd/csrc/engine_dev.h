@@ -90,6 +90,7 @@ struct SlotDev {  // one number being operated on (one test in a batch)
     double kscale;              // k / (4H)
     double rscale;              // 1 / R
     uint64_t k;
+    uint64_t kinv;              // floor((2^64 - 1) / k), for the fold's division by k
     uint32_t count;             // the slot takes part in launches with step < count
     int b0;                     // digits have b0 or b0+1 bits
     int top_i0, top_n, top_m, top_shift;  // fold: first top digit within the last chunk, count,

@@ -320,18 +320,27 @@ __device__ __forceinline__ const SlotDev &pick_slot(const SlotDev *slots, const 
     return slots ? slots[i] : one;
 }
 
-// floor(h / k) for 0 < k < 2^32
-__device__ __forceinline__ i128 floordiv_small(i128 h, uint32_t k) {
+// floor(t / k) for t < 2^64, 0 < k < 2^32, inv = floor((2^64 - 1) / k): inv >= (2^64 - k) / k, so
+// the estimate hi64(t inv) is floor(t/k) or one less; one correction step (Barrett reduction)
+__device__ __forceinline__ uint64_t divk(uint64_t t, uint32_t k, uint64_t inv, uint64_t &r) {
+    uint64_t q = __umul64hi(t, inv);
+    r = t - q * k;
+    if (r >= k) {
+        q++;
+        r -= k;
+    }
+    return q;
+}
+
+// floor(h / k) for 0 < k < 2^32, by 64/32-bit long division in base 2^32 (each step's dividend
+// (r << 32 | next word) is below k 2^32 <= 2^64)
+__device__ __forceinline__ i128 floordiv_small(i128 h, uint32_t k, uint64_t inv) {
     const bool neg = h < 0;
     u128 a = neg ? (u128)(-h) : (u128)h;
-    uint64_t hi = (uint64_t)(a >> 64), lo = (uint64_t)a;
-    uint64_t qh = hi / k, r = hi % k;
-    uint64_t t = (r << 32) | (lo >> 32);
-    uint64_t q1 = t / k;
-    r = t % k;
-    t = (r << 32) | (lo & 0xffffffffull);
-    uint64_t q0 = t / k;
-    r = t % k;
+    uint64_t hi = (uint64_t)(a >> 64), lo = (uint64_t)a, r;
+    const uint64_t qh = divk(hi, k, inv, r);
+    const uint64_t q1 = divk((r << 32) | (lo >> 32), k, inv, r);
+    const uint64_t q0 = divk((r << 32) | (lo & 0xffffffffull), k, inv, r);
     u128 q = ((u128)qh << 64) | ((u128)q1 << 32) | q0;
     if (!neg) return (i128)q;
     return -(i128)q - (r ? 1 : 0);
@@ -350,7 +359,7 @@ __device__ __forceinline__ int64_t fold_top_dev(const SlotDev &s, int32_t *d, in
     }
     const i128 K = (i128)s.k;
     const i128 lo = Hh & (((i128)1 << m) - 1), hi = Hh >> m;
-    i128 q = floordiv_small(hi, (uint32_t)s.k);
+    i128 q = floordiv_small(hi, (uint32_t)s.k, s.kinv);
     i128 top = lo + ((hi - q * K) << m);
     if (2 * top >= (K << m)) {
         top -= K << m;
