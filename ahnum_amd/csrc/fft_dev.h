@@ -206,6 +206,41 @@ template <bool INV> struct Dft<3, INV> : DftOdd<3, INV> {};
 template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
 template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
 
+// Composite lengths N1 N2 with coprime factors by the prime-factor (Good-Thomas) mapping, no twiddle
+// multiplications: input n = (N2 n1 + N1 n2) mod N, output k = (N2 q2 k1 + N1 q1 k2) mod N with
+// q2 = N2^-1 mod N1, q1 = N1^-1 mod N2; then n k = N2 q2 N2 n1 k1 + N1 q1 N1 n2 k2 = N2 n1 k1 +
+// N1 n2 k2 (mod N), so the DFT is a DFT-N1 over n1 of DFT-N2s over n2.
+__host__ __device__ constexpr int inv_mod(int a, int m) {
+    for (int q = 1; q < m; q++)
+        if ((a * q) % m == 1) return q;
+    return 1;
+}
+template <int N1, int N2, bool INV> struct DftPfa {
+    static __device__ __forceinline__ void run(double2 *v) {
+        constexpr int N = N1 * N2, Q2 = inv_mod(N2 % N1, N1), Q1 = inv_mod(N1 % N2, N2);
+        double2 t[N1][N2];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+#pragma unroll
+            for (int n2 = 0; n2 < N2; n2++) t[n1][n2] = v[(N2 * n1 + N1 * n2) % N];
+            Dft<N2, INV>::run(t[n1]);
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+            double2 u[N1];
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) u[n1] = t[n1][k2];
+            Dft<N1, INV>::run(u);
+#pragma unroll
+            for (int k1 = 0; k1 < N1; k1++) v[(N2 * Q2 * k1 + N1 * Q1 * k2) % N] = u[k1];
+        }
+    }
+};
+template <bool INV> struct Dft<6, INV> : DftPfa<2, 3, INV> {};
+template <bool INV> struct Dft<10, INV> : DftPfa<2, 5, INV> {};
+template <bool INV> struct Dft<12, INV> : DftPfa<4, 3, INV> {};
+template <bool INV> struct Dft<14, INV> : DftPfa<2, 7, INV> {};
+
 // ---- Stockham stages in LDS -----------------------------------------------------------------
 // `how` sequences of length n; element (seq, idx) at buf[idx*is + seq*ss]. A stage of radix P
 // after Ns points have been combined: butterfly j reads idx j + r*n/P, twiddles input r by
@@ -703,11 +738,20 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
 // outputs to st(it, r, row, col, v) instead of the buffer (STOUT): a load or store phase fused into
 // the FFT saves a pass through LDS and its barriers. (it, r) name the thread's iteration and the
 // butterfly input/output, (row, col) the element.
+// Radix plan of fft_io: two stages (N/8, then 8) when N/8 is a length of Dft, which halves the
+// LDS passes and barriers of the column lengths up to 128; else the plan of the other kernels.
+__host__ __device__ constexpr bool io_two(int n) {
+    return n % 8 == 0 && (n / 8 == 2 || n / 8 == 3 || n / 8 == 4 || n / 8 == 5 || n / 8 == 6 || n / 8 == 7 ||
+                          n / 8 == 8 || n / 8 == 10 || n / 8 == 12 || n / 8 == 14 || n / 8 == 16);
+}
+__host__ __device__ constexpr int io_nstages(int n) { return io_two(n) ? 2 : ct_nstages(n); }
+__host__ __device__ constexpr int io_radix(int n, int idx) { return io_two(n) ? (idx == 0 ? n / 8 : 8) : ct_radix(n, idx); }
+
 template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, class LD, class ST>
 __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST &st) {
-    constexpr int NST = ct_nstages(N);
+    constexpr int NST = io_nstages(N);
     if constexpr (I < NST) {
-        constexpr int P = ct_radix(N, I), NB = N / P, TOTAL = NB * HOW;
+        constexpr int P = io_radix(N, I), NB = N / P, TOTAL = NB * HOW;
         constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
         constexpr bool FROM_LD = LDIN && I == 0, TO_ST = STOUT && I == NST - 1;
         double2 v[ITER][P];
@@ -761,7 +805,7 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
 
 // The last stage's thread mapping of fft_io (for prefetching what its outputs need).
 template <int N, int HOW> struct LastStage {
-    static constexpr int P = ct_radix(N, ct_nstages(N) - 1), NB = N / P, TOTAL = NB * HOW;
+    static constexpr int P = io_radix(N, io_nstages(N) - 1), NB = N / P, TOTAL = NB * HOW;
     static constexpr int ITER = (TOTAL + kNT - 1) / kNT;
 };
 
@@ -1320,7 +1364,9 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
     for (int it = 0; it < ITSC; it++) {
         const int w = it * kNT + (int)threadIdx.x;
         if (NSC % kNT != 0 && w >= NSC) continue;
-        const int r = w / NSUB, q = w - r * NSUB;
+        // sub-chunk (r, q) is w = q R + r: the lanes of a wave take consecutive rows, so their
+        // reads of A (row stride G + 1) fall in distinct LDS banks
+        const int q = w / R, r = w - q * R;
         const double tap = WA[r];
         double cy = 0;
         bg[it] = 0;
@@ -1370,10 +1416,10 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
     for (int it = 0; it < ITSC; it++) {
         const int w = it * kNT + (int)threadIdx.x;
         if (NSC % kNT != 0 && w >= NSC) continue;
-        const int r = w / NSUB, q = w - r * NSUB;
+        const int q = w / R, r = w - q * R;
         int64_t cy;
         if (q > 0) {
-            cy = SC[w - 1];
+            cy = SC[w - R];
         } else {
             const int pidx = g > 0 ? (g - 1) * R + r : (r > 0 ? (NG - 1) * R + r - 1 : (NG - 1) * R + R - 1);
             cy = wait_flag(s.flags + pidx, seq, s.maxerr);
