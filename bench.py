@@ -22,6 +22,7 @@ two-pass backend (--backend c2p) timed on the host cores, one test per core.
 """
 import argparse
 import json
+import math
 import multiprocessing as mp
 import os
 import sys
@@ -31,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA counted as two)
 
 
 def cpu_worker(args):
@@ -238,6 +240,8 @@ def main():
     alg_bytes = kbytes / klaunch
     achieved = alg_bytes / avg_s / 1e9
     traffic = load_traffic(kname)
+    step_bytes = sum(32 * L for L in Ls)
+    step_flops = sum(5 * L * math.log2(L / 2) + 20 * L for L in Ls)
 
     out = {
         "metric": "tests/hour at n~660K (Fermat base 3, FFT 49152, Gerbicz-checkable residues), batched",
@@ -279,6 +283,17 @@ def main():
             "fft_len": L0,
             "maxerr": round(plain_err, 4),
             "number": f"{k0}*2^{n0}-1",
+        },
+        # the whole step against both rooflines, in the algorithmic units of a squaring: 32 L bytes
+        # (two passes, each reading and writing the 8L-byte residue) and 5 L log2(L/2) + 20 L flops
+        # (forward and inverse complex FFTs of L/2 points, split square and carry/weights)
+        "path_roofline": {
+            "bytes_per_step": int(step_bytes),
+            "flops_per_step": int(step_flops),
+            "mem_achieved_gbs": round(step_bytes * a.steps / wall / 1e9, 1),
+            "mem_frac": round(step_bytes * a.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+            "fp64_achieved_tflops": round(step_flops * a.steps / wall / 1e12, 2),
+            "fp64_frac": round(step_flops * a.steps / wall / 1e12 / FP64_PEAK_TFLOPS, 4),
         },
         "batched_us_per_test_squaring": round(wall * 1e6 / a.steps / a.batch, 4),
         "batched_vs_plain": round(tph / plain_tph, 2),
