@@ -5,8 +5,9 @@
   python bench.py --serve [--serve-count C] [--serve-rate R] [--serve-horizon S] ...
 
 A step is one squaring of every residue in the batch (B Fermat base-3 tests with k odd in
-[1001, 9999], n in [650000, 670000], seeded): the batched engine advances all B tests with one
-launch per pass. The same number is also timed alone through the kite.h path (the plain
+[1001, 9999], n in [650000, 670000], seeded; B = 384 by default, the batch size that measured
+fastest on one MI355X among 256-768): the batched engine advances all B tests with one launch
+per pass and stream. The same number is also timed alone through the kite.h path (the plain
 one-test-at-a-time GPU path). tests/hour follows the reference's throughput formula
 (tools/kite.c:262): sum over tests of squarings/s * 3600 / n.
 
@@ -14,8 +15,9 @@ The timed region runs square calls of 1000 steps (a Gerbicz block), each replaye
 graph the batch captured for it. Prints one JSON line. `roofline` is for the dominant pass kernel
 (the most device time), timed right after in a second region of 1000 steps launched directly, with
 HIP events on the batch stream around each launch of every 10th step;
-algorithmic bytes per launch are 16 L per test for the row pass (read + write of the 8L-byte
-residue) and 8 L per test for each column pass (32 L per squaring in total). `traffic` is the
+algorithmic bytes per launch are 16 L per test for the row pass and for the fused column pass
+(each reads and writes the 8L-byte residue; 32 L per squaring in total). `path_roofline` puts the
+whole step against both rooflines. `traffic` is the
 kernel's HBM bytes per launch from profiles/hbm_traffic_single_large_batch.txt (rocprofv3
 FETCH_SIZE and WRITE_SIZE runs of this benchmark, calibrated by a copy of known size). `cpu_baseline` is the oracle's restatement of the reference's portable
 two-pass backend (--backend c2p) timed on the host cores, one test per core.
@@ -138,7 +140,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=384)
     ap.add_argument("--plain-steps", type=int, default=3000)
     ap.add_argument("--cpu-secs", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
