@@ -97,3 +97,60 @@ def test_sigterm_checkpoint_and_resume(tmp_path):
     m = re.search(r"restarting at (\d+\.\d)%\.", out)
     assert m and float(m.group(1)) > 0
     assert f"{expr} is not prime. RES64: BDD48C45E17CDD83" in out
+
+
+# tests/prst_compat.sh: PRST 14.0.1888's files for 1203*2^90001-1 -d -fermat a 3 -proof save 512
+# keep, then -proof build 512, then -proof cert (MD5s of the files; RES64s of the lines)
+PRST_MD5 = {
+    "prst_1107603830.proof.0": "dafcb9cf37507263eaee0ece52991b3e",
+    "prst_1107603830.proof.512": "c57ae2618fb59c8652987543ab7cec29",
+    "prst_1107603830.prod.0": "57b767aa53d89f86a5fe742d66089abe",
+    "prst_1107603830.prod.1": "2d426d68ba20d327d66d58a1e0c33316",
+    "prst_1107603830.prod.2": "c3b3acd901f61818ad42b61337cadf9b",
+    "prst_1107603830.prod.3": "bfcbaed0176cd4a985744b04692d21f8",
+    "prst_1107603830.prod.4": "a58741b2b37270b7f213119e48d9ae4a",
+    "prst_1107603830.prod.5": "20d54b4ed93bd466f69c39a51a27113b",
+    "prst_1107603830.prod.6": "b8f2e1c7af03a5d277007f944335876e",
+    "prst_1107603830.prod.7": "1d8bcc840e8d7c3246b17898d1b96b34",
+    "prst_1107603830.prod.8": "3d38d7470434544fa95858a251bd3b8a",
+}
+PRST_CERT_MD5 = "f3ae012634454e38aa7311deccccde56"
+
+
+def md5_of(path):
+    import hashlib
+    return hashlib.md5(open(path, "rb").read()).hexdigest()
+
+
+def test_proof_files_match_prst_byte_for_byte(tmp_path):
+    """Proof points, products and certificate identical to PRST's (by MD5), the raw and final
+    certificate RES64s PRST prints, and the certificate verifying in a fresh directory."""
+    expr = "1203*2^90001-1"
+    r, out = run([expr, "-d", "-fermat", "a", "3", "-proof", "save", "512", "keep"], tmp_path, timeout=600)
+    assert r.returncode == 0, r.stderr
+    assert f"{expr} Saving 512 proof points." in out
+    assert "is not prime. RES64: 4BE9F1953E350085" in out
+    assert re.search(rf"{re.escape(expr)} compressed 512 points to 9 products, time: [0-9.]+ s\.", out)
+    assert "raw certificate RES64: EC81DC6893506908" in out
+    got = {f: md5_of(tmp_path / f) for f in PRST_MD5}
+    assert got == PRST_MD5
+    for f in PRST_MD5:  # every file carries its MD5 sidecar
+        assert (tmp_path / (f + ".md5")).read_text() == PRST_MD5[f]
+    assert not [f for f in os.listdir(tmp_path) if f.startswith("kite_") and f.endswith(".done")]
+
+    r, out = run([expr, "-d", "-fermat", "a", "3", "-proof", "build", "512"], tmp_path, timeout=600)
+    assert r.returncode == 0, r.stderr
+    assert f"{expr} Building certificate from 9 products." in out
+    assert "certificate RES64: EC81DC6893506908" in out
+    assert md5_of(tmp_path / "prst_1107603830.cert") == PRST_CERT_MD5
+    assert not (tmp_path / "prst_1107603830.proof.0").exists()  # removed without keep
+
+    v = tmp_path / "v"
+    v.mkdir()
+    (v / "verify.cert").write_bytes((tmp_path / "prst_1107603830.cert").read_bytes())
+    r, out = run([expr, "-d", "-proof", "cert", "verify.cert"], v, timeout=600)
+    assert r.returncode == 0, r.stderr
+    assert re.search(rf"Verifying certificate of {re.escape(expr)}, complexity = \d+\.", out)
+    assert "certificate RES64: EC81DC6893506908" in out
+    lines = (tmp_path / "result.txt").read_text().splitlines()
+    assert any("raw certificate RES64: EC81DC6893506908" in x for x in lines)

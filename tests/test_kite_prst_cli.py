@@ -1,6 +1,6 @@
 """kite-prst, the PRST-compatible command line (reference tools/kite_prst.c): argument parsing, the
-hand-off to PRST for everything outside its slice, and -v. These run without a GPU: the hand-off
-happens before the engine is touched."""
+hand-off to PRST for everything outside its slice, the proof count check, and -v. These run without
+a GPU: the hand-off happens before the engine is touched."""
 import os
 import stat
 import subprocess
@@ -41,8 +41,8 @@ def test_usage_without_a_number():
 
 
 @pytest.mark.parametrize("args", [
-    ["9357*2^20000-1", "-d", "-fermat", "a", "3", "-proof", "save", "512"],  # proofs are PRST's here
-    ["9357*2^20000-1", "-d", "-proof", "cert", "verify.cert"],
+    ["9357*2^20000-1", "-d", "-proof", "save", "512"],                       # a proof of an LLR run
+    ["9357*2^20000-1", "-d", "-fermat", "a", "3", "-proof", "merge", "512"],  # no such proof step here
     ["9357*2^20000+1", "-d"],                                                 # not k*2^n-1
     ["9358*2^20000-1", "-d", "-fermat", "a", "3"],                            # k even
     ["9357*2^20000-1", "-d", "-fermat", "a", "3", "-oddoption"],
@@ -61,3 +61,9 @@ def test_missing_prst_is_an_error(tmp_path):
     r = run(["9357*2^20000+1", "-d"], env={"KITE_PRST": str(tmp_path / "nowhere")})
     assert r.returncode == 2
     assert "cannot run PRST" in r.stderr
+
+
+def test_proof_count_must_be_a_power_of_two():
+    r = run(["9357*2^20000-1", "-d", "-fermat", "a", "3", "-proof", "save", "384"])
+    assert r.returncode == 1
+    assert "proof count is not a power of 2." in r.stderr
