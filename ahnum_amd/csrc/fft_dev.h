@@ -1018,17 +1018,21 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &
     const int km = (side == 1 && !special) ? 15 - t : t;  // k1 held after the transpose
     double *X = xs + g * 256;
     double2 *xr = s.x + (size_t)(act ? row : 0) * C + t;
-    const double2 *tw16 = sh.tw16;
+    // the row twiddles w_256^{t j} ([j][t], 4 KB) in LDS: L2 round trips in the FFTs cost more
+    __shared__ double2 tws[256];
+    for (int i = lane; i < 256; i += (int)blockDim.x) tws[i] = sh.tw16[i];
     double2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = act ? gld(xr + 16 * i) : make_double2(0, 0);
+    __syncthreads();
+    const double2 *tw16 = tws;
     // forward: DFT16 over a, twiddle w_256^{t k1}, transpose, DFT16 over b
     Dft<16, false>::run(v);
-    // twiddles w_256^{t j} from the [j][t] table, four loads in flight at a time (register pressure)
+    // twiddles w_256^{t j} from the [j][t] table
 #pragma unroll
     for (int j = 1; j < 16; j++) {
         if (j % 4 == 1) __builtin_amdgcn_sched_barrier(0);
-        v[j] = cmul(v[j], gld(tw16 + 16 * j + t));
+        v[j] = cmul(v[j], tw16[16 * j + t]);
     }
     xpose_ri(v, X, [&](int j) { return j * 16 + (t ^ j); }, [&](int j) { return km * 16 + (j ^ km); });
     Dft<16, false>::run(v);  // v[k2] = U[row][km + 16 k2]
@@ -1070,7 +1074,7 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &
 #pragma unroll
     for (int j = 1; j < 16; j++) {
         if (j % 4 == 1) __builtin_amdgcn_sched_barrier(0);
-        v[j] = cmulc(v[j], gld(tw16 + 16 * j + km));
+        v[j] = cmulc(v[j], tw16[16 * j + km]);
     }
     xpose_ri(v, X, [&](int j) { return j * 16 + (km ^ j); }, [&](int j) { return t * 16 + (j ^ t); });
     Dft<16, true>::run(v);

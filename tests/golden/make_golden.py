@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Writes the GMP golden fixtures under tests/golden/ (run in the build container, not on a GPU box).
+"""Writes the GMP golden fixtures under tests/golden/ (needs a host with GMP and a C compiler; no GPU).
 
   python tests/golden/make_golden.py plain_small     # 16 seeded n~65K tests: RES64 by GMP
   python tests/golden/make_golden.py chain           # x_i = 3^(k*2^i) mod N at chosen i
