@@ -113,6 +113,21 @@ def test_batch_prp_primecrunch_rows(golden):
         assert r.maxerr < 0.4
 
 
+def test_batch_reference_vectors_at_full_size(golden):
+    """The reference's own full-size vectors: 9357*2^650001-1 (tools/kite-bench.sh:28-29) and the
+    prime 9001*2^662695-1 (tests/benchmarks.csv:2), at FFT length 49152, in one batch."""
+    rows = [r for r in golden("single_vectors.csv") if int(r[1]) >= 600000]
+    assert len(rows) == 2
+    tests = [(int(k), int(n)) for k, n, _, _ in rows]
+    say("batch of", tests)
+    rc, out = kite.batch_prp(tests, block=1000)
+    assert rc == 0
+    for (k, n, want, _), (st, r) in zip(rows, out):
+        assert st == 0 and "%016X" % r.res64 == want, (k, n)
+        assert r.fft_len == 49152 and r.maxerr < 0.4
+    assert out[1][1].is_prp
+
+
 def test_batch_subset_vs_gmp(golden):
     """Seeded tests of the 256-test mixed batch (k odd in [3, 9999], n in [300K, 700K]) whose GMP
     RES64s are committed (tests/golden/res64_batch_subset.csv)."""
