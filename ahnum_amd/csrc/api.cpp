@@ -24,7 +24,7 @@ extern "C" kite_ctx *kite_ctx_new(uint64_t k, uint64_t n, int32_t c, const kite_
         return nullptr;
     }
     ctx->plan_order = o && o->plan_order ? o->plan_order - 1 : kite_tuned_order(ctx->L);
-    ctx->nlimbs = (n + 64 + 63) / 64;
+    ctx->nlimbs = limbs_for_n(n);
     ctx->Nl.assign(ctx->nlimbs, 0);
     ctx->tmpdig.assign(ctx->L, 0);
     if (build_layout(ctx)) {

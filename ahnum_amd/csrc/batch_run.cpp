@@ -9,7 +9,9 @@
 // (kite_batch_set_xadd), a round-off check after every block with a device copy of the last
 // verified state, and a composite answer confirmed at the next FFT length (a third run breaks a
 // tie). Jobs start once they have arrived and the slot and HBM limits allow; a finished job's
-// slot goes to the next one at once (continuous batching).
+// slot goes to the next one at once (continuous batching). With a checkpoint directory a job
+// resumes from its verified state on file and writes it back as it goes (src/prp.c:156-212 and
+// src/llr.c:92-146 for the two file formats).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -17,6 +19,7 @@
 #include <cstring>
 #include <ctime>
 #include <deque>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -48,6 +51,10 @@ struct Job {
     // Fermat
     uint64_t full = 0, i = 0, vi = 0, check_at = 0;
     std::vector<uint64_t> vx, vd;
+    // start of the Gerbicz product (x0 = base^k at iteration 0, or a verified x restored from a
+    // checkpoint at iteration x0_at); blocks of B squarings count from there
+    std::vector<uint64_t> x0;
+    uint64_t x0_at = 0;
     uint32_t consecutive_fail = 0, tail_attempt = 0;
     // LLR
     uint32_t P = 0;
@@ -59,6 +66,7 @@ struct Job {
     kite_job_result res{};
     int rc = 0;
     bool started = false;
+    double last_ck = 0;  // when its checkpoint was last written
 };
 
 void power_start(kite_ctx *ctx, kite_num *x0, uint64_t k, uint32_t base) {
@@ -92,7 +100,8 @@ int start_fermat(kite_batch *b, Job &t, uint32_t L, uint32_t B, uint32_t M) {
     kite_ctx *ctx = kite_batch_ctx(b, t.slot);
     t.L = kite_fft_len(ctx);
     kite_num *x0 = kite_batch_num(b, t.slot, kX0);
-    power_start(ctx, x0, t.k, t.base);
+    if (t.x0.empty()) power_start(ctx, x0, t.k, t.base);
+    else kite_set_limbs(ctx, x0, t.x0.data());
     if (t.vx.empty()) {
         t.vx.assign(ctx->nlimbs, 0);
         kite_get_limbs(ctx, x0, t.vx.data());
@@ -102,7 +111,7 @@ int start_fermat(kite_batch *b, Job &t, uint32_t L, uint32_t B, uint32_t M) {
     kite_set_limbs(ctx, kite_batch_num(b, t.slot, kX), t.vx.data());
     kite_set_limbs(ctx, kite_batch_num(b, t.slot, kD), t.vd.data());
     t.i = t.vi;
-    t.full = t.n / B * B;
+    t.full = t.x0_at + (t.n - t.x0_at) / B * B;
     t.check_at = std::min(t.vi + (uint64_t)B * M, t.full);
     t.phase = t.vi >= t.full ? kTailX : kBlocks;
     if (t.phase == kTailX) {
@@ -158,16 +167,91 @@ bool nums_equal(kite_batch *b, int slot, int a, int c, std::vector<uint64_t> &la
     return la == lb;
 }
 
+// Name of job t's checkpoint file: kite_<N mod 3417905339>.<base>.ckpt (Fermat) or
+// kite_<N mod 3417905339>.llr.ckpt (LLR), as kite-prst names them (tools/kite_prst.c).
+std::string ckpt_name(const Job &t) {
+    char name[64];
+    const unsigned long long fp = n_mod(t.k, t.n, 3417905339ull);
+    if (t.kind == KITE_JOB_LLR) snprintf(name, sizeof name, "kite_%llu.llr.ckpt", fp);
+    else snprintf(name, sizeof name, "kite_%llu.%u.ckpt", fp, t.base);
+    return name;
+}
+
 struct Runner {
     kite_batch *b = nullptr;
     kite_run_opts o{};
     uint32_t B = 1000, M = 250;
-    double maxround = 0.40, t0 = 0;
+    double maxround = 0.40, t0 = 0, ck_secs = 600;
+    std::string dir;  // checkpoint directory ("" = no checkpoints)
     std::vector<Job> all;
     uint64_t bytes_in_flight = 0;
     int rc_all = 0;
 
+    std::string ckpt_path(const Job &t) const { return dir + "/" + ckpt_name(t); }
+
+    // Restores job t from its checkpoint file, if a valid one exists. A Fermat state is trusted
+    // for its x only (verified when written): the Gerbicz product restarts at x, so files from
+    // writers with other block sizes resume too. An LLR file restores the run, its step and the
+    // earlier runs' results; a confirmation run keeps the length it was written at.
+    void resume(Job &t) {
+        const size_t nl = limbs_for_n(t.n);
+        const std::string path = ckpt_path(t);
+        std::vector<uint64_t> x(nl), d(nl);
+        uint32_t L = 0;
+        if (t.kind == KITE_JOB_PRP) {
+            uint64_t it = 0;
+            if (prp_checkpoint_read(path.c_str(), t.k, t.n, t.base, &it, &L, x.data(), d.data(), nl) || it > t.n)
+                return;
+            t.vi = t.x0_at = it;
+            t.vd = t.x0 = x;
+            t.vx.swap(x);
+        } else {
+            LlrCheckpoint c;
+            if (llr_checkpoint_read(path.c_str(), t.k, t.n, &c, x.data(), nl) || c.P != t.P) return;
+            t.run = c.run;
+            for (int r = 0; r < 2; r++) {
+                t.lres[r] = c.prev[r];
+                t.lzero[r] = r < c.run && c.prev[r] == 0;
+            }
+            t.res.confirm_runs = (uint32_t)c.run;
+            t.vi = c.iter;
+            t.vx.swap(x);
+            L = c.L;
+            if (c.run > 0) t.L = L;
+        }
+        if (!t.L && L > kite_choose_fft_len(t.k, t.n)) t.L = L;  // a round-off retry had lengthened it
+        t.res.resumed = 1;
+        t.res.resumed_at = t.vi;
+    }
+
+    // Writes Fermat job t's verified state: x at iteration vi, with d = x, which tells a reader
+    // to restart the Gerbicz product at x.
+    void save_fermat(Job &t) {
+        if (dir.empty() || t.vi == 0) return;
+        prp_checkpoint_write(ckpt_path(t).c_str(), t.k, t.n, t.base, t.vi, t.L, t.vx.data(), t.vx.data(), t.vx.size());
+        t.last_ck = now_s();
+    }
+
+    // Writes LLR job t's state: x (in its slot, at step vi of the current run) after a sync.
+    void save_llr(Job &t, std::vector<uint64_t> &buf) {
+        if (dir.empty()) return;
+        kite_ctx *ctx = kite_batch_ctx(b, t.slot);
+        buf.resize(ctx->nlimbs);
+        kite_batch_sync(b);
+        kite_get_limbs(ctx, kite_batch_num(b, t.slot, kX), buf.data());
+        LlrCheckpoint c;
+        c.iter = t.vi;
+        c.P = t.P;
+        c.L = t.L;
+        c.run = t.run;
+        c.prev[0] = t.lres[0];
+        c.prev[1] = t.lres[1];
+        llr_checkpoint_write(ckpt_path(t).c_str(), t.k, t.n, c, buf.data(), buf.size());
+        t.last_ck = now_s();
+    }
+
     void done(Job &t, int rc) {
+        if (!dir.empty()) remove(ckpt_path(t).c_str());
         if (t.slot >= 0) kite_batch_remove(b, t.slot);
         t.slot = -1;
         t.phase = kDone;
@@ -277,6 +361,8 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
     R.B = R.o.block ? R.o.block : 1000;
     R.M = R.o.gerbicz_blocks ? R.o.gerbicz_blocks : std::max<uint32_t>(R.B / 4, 8);
     R.maxround = R.o.max_roundoff > 0 ? R.o.max_roundoff : 0.40;
+    if (R.o.checkpoint_dir && R.o.checkpoint_dir[0]) R.dir = R.o.checkpoint_dir;
+    R.ck_secs = R.o.checkpoint_secs > 0 ? R.o.checkpoint_secs : 600;
     const size_t max_slots = R.o.slots ? R.o.slots : (size_t)-1;
     R.b = kite_batch_new();
     if (!R.b) return -1;
@@ -301,6 +387,7 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
     std::deque<size_t> waiting(order.begin(), order.end());
     std::vector<size_t> active;
     std::vector<uint64_t> la, lb;
+    bool engine_failed = false;
     R.t0 = now_s();
     auto admit = [&]() {
         const double now = now_s() - R.t0;
@@ -326,6 +413,8 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
                     continue;
                 }
             }
+            if (!R.dir.empty()) R.resume(t);
+            t.last_ck = now_s();
             const int r = R.start(t, t.L);
             if (r) {
                 R.done(t, r < 0 ? r : -1);
@@ -406,6 +495,7 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
         }
         if (kite_batch_square(b, which.data(), cnt.data())) {
             if (!R.rc_all) R.rc_all = -1;
+            engine_failed = true;
             break;
         }
         // testing hook: a +1 digit corruption at the end of the block holding inject_fault_at
@@ -449,6 +539,7 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
                     R.llr_run_done(t);
                     if (t.phase != kDone) still.push_back(ti);
                 } else {
+                    if (now_s() - t.last_ck >= R.ck_secs) R.save_llr(t, la);
                     still.push_back(ti);
                 }
                 break;
@@ -464,6 +555,7 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
                     t.vd = lb;
                     t.vi = t.i;
                     t.consecutive_fail = 0;
+                    if (now_s() - t.last_ck >= R.ck_secs && t.vi < t.n) R.save_fermat(t);
                     if (t.vi >= t.full) {
                         if (t.full == t.n) {
                             R.finish_fermat(t, t.vx, 0);
@@ -515,6 +607,14 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
         }
         active.swap(still);
         admit();
+    }
+    if (!R.dir.empty() && !engine_failed) {  // stopped by the horizon: save the jobs in flight
+        for (size_t ti : active) {
+            Job &t = R.all[ti];
+            if (t.phase == kDone || t.slot < 0) continue;
+            if (t.kind == KITE_JOB_LLR) R.save_llr(t, la);
+            else R.save_fermat(t);
+        }
     }
     for (size_t i = 0; i < njobs; i++) {
         Job &t = R.all[i];

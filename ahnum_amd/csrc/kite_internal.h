@@ -84,6 +84,26 @@ const char *engine_last_error();
 // ---- LLR helpers (llr.cpp) -------------------------------------------------------------------
 // x = V_k(P, -1) by the Lucas ladder (cP holds P, c2 holds 2; v1, t are scratch).
 void llr_lucas_vk(kite_ctx *ctx, uint64_t k, kite_num *x, kite_num *v1, kite_num *cP, kite_num *c2, kite_num *t);
+uint64_t n_mod(uint64_t k, uint64_t n, uint64_t m);  // (k 2^n - 1) mod m
+
+// ---- checkpoint files (prp.cpp, llr.cpp) -----------------------------------------------------
+inline size_t limbs_for_n(uint64_t n) { return (n + 64 + 63) / 64; }  // limbs of a residue, k < 2^64
+// KITECKP1: Fermat state (x, d) at iteration iter, written at length L; 0 on success. The reader
+// checks k, n, base, the limb count and the checksum.
+int prp_checkpoint_write(const char *path, uint64_t k, uint64_t n, uint32_t base, uint64_t iter, uint32_t L,
+                         const uint64_t *x, const uint64_t *d, size_t nl);
+int prp_checkpoint_read(const char *path, uint64_t k, uint64_t n, uint32_t base, uint64_t *iter, uint32_t *L,
+                        uint64_t *x, uint64_t *d, size_t nl);
+// KITELLR1: LLR run `run`'s x at step iter (length L) and the low words of the earlier runs' results.
+struct LlrCheckpoint {
+    uint64_t iter = 0;
+    uint32_t P = 0, L = 0;
+    int run = 0;
+    uint64_t prev[2] = {0, 0};
+};
+int llr_checkpoint_write(const char *path, uint64_t k, uint64_t n, const LlrCheckpoint &c, const uint64_t *x,
+                         size_t nl);
+int llr_checkpoint_read(const char *path, uint64_t k, uint64_t n, LlrCheckpoint *c, uint64_t *x, size_t nl);
 
 }  // namespace kite
 

@@ -28,12 +28,6 @@ double now_s() {
 
 uint64_t mulmod(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)((unsigned __int128)a * b % m); }
 
-uint64_t n_mod(uint64_t k, uint64_t n, uint64_t m) {  // (k 2^n - 1) mod m
-    uint64_t r = 1 % m, b = 2 % m;
-    for (uint64_t e = n; e; e >>= 1, b = mulmod(b, b, m))
-        if (e & 1) r = mulmod(r, b, m);
-    return (mulmod(k % m, r, m) + m - 1 % m) % m;
-}
 
 int jacobi(uint64_t a, uint64_t m) {  // m odd
     int r = 1;
@@ -236,6 +230,30 @@ int llr_run(uint64_t k, uint64_t n, uint32_t P, const kite_llr_opts *o, uint32_t
 }  // namespace
 
 namespace kite {
+
+uint64_t n_mod(uint64_t k, uint64_t n, uint64_t m) {
+    uint64_t r = 1 % m, b = 2 % m;
+    for (uint64_t e = n; e; e >>= 1, b = mulmod(b, b, m))
+        if (e & 1) r = mulmod(r, b, m);
+    return (mulmod(k % m, r, m) + m - 1 % m) % m;
+}
+
+int llr_checkpoint_write(const char *path, uint64_t k, uint64_t n, const LlrCheckpoint &c, const uint64_t *x,
+                         size_t nl) {
+    return ckpt_write(path, make_hdr(k, n, c.P, c.iter, nl, c.L, c.run, c.prev), x);
+}
+
+int llr_checkpoint_read(const char *path, uint64_t k, uint64_t n, LlrCheckpoint *c, uint64_t *x, size_t nl) {
+    LlrHdr h;
+    if (ckpt_read(path, &h, x, nl) || h.k != k || h.n != n || h.run > 2 || h.iter > n - 1) return -1;
+    c->iter = h.iter;
+    c->P = (uint32_t)h.P;
+    c->L = (uint32_t)h.fft_len;
+    c->run = (int)h.run;
+    c->prev[0] = h.prev[0];
+    c->prev[1] = h.prev[1];
+    return 0;
+}
 
 // V_k(P, -1) by the ladder (V_m, V_{m+1}) over the bits of k (src/llr.c:150-184).
 void llr_lucas_vk(kite_ctx *ctx, uint64_t k, kite_num *x, kite_num *v1, kite_num *cP, kite_num *c2, kite_num *t) {

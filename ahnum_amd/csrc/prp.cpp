@@ -175,6 +175,20 @@ void power_start(State &s, uint64_t k, uint32_t base, uint64_t *l) {
 
 }  // namespace
 
+namespace kite {
+
+int prp_checkpoint_write(const char *path, uint64_t k, uint64_t n, uint32_t base, uint64_t iter, uint32_t L,
+                         const uint64_t *x, const uint64_t *d, size_t nl) {
+    return ckpt_write(path, k, n, base, iter, L, x, d, nl);
+}
+
+int prp_checkpoint_read(const char *path, uint64_t k, uint64_t n, uint32_t base, uint64_t *iter, uint32_t *L,
+                        uint64_t *x, uint64_t *d, size_t nl) {
+    return ckpt_read(path, k, n, base, iter, L, x, d, nl);
+}
+
+}  // namespace kite
+
 extern "C" int kite_prp_checkpoint_iter(const char *path, uint64_t k, uint64_t n, uint32_t base, uint64_t *iter) {
     FILE *f = fopen(path, "rb");
     if (!f) return -1;
@@ -237,7 +251,27 @@ extern "C" int kite_prp(uint64_t k, uint64_t n, uint32_t base, const kite_prp_op
             power_start(s, k, base, la.data());
         }
     }
-    const uint64_t full = n / B * B;
+    // A verified checkpoint with d == x (kite_batch_run writes them so; a product from base^k
+    // has d != x past iteration 0) restarts the Gerbicz product at that x: it becomes x0, and
+    // blocks count from its iteration. Every verified write then restarts the product again, so
+    // the file stays in that form and an unverified file chains from it.
+    std::vector<uint64_t> x0l;
+    uint64_t x0_at = 0;
+    if (vi > 0 && vx == vd) {
+        x0l = vx;
+        x0_at = vi;
+        kite_set_limbs(s.ctx, s.x0, x0l.data());
+    }
+    auto write_verified = [&]() {
+        if (!x0l.empty()) {
+            x0l = vx;
+            vd = vx;
+            kite_set_limbs(s.ctx, s.x0, x0l.data());
+            if (o.checkpoint_unverified) remove(o.checkpoint_unverified);
+        }
+        ckpt_write(o.checkpoint, k, n, base, vi, kite_fft_len(s.ctx), vx.data(), vd.data(), nl);
+    };
+    const uint64_t full = x0_at + (n - x0_at) / B * B;
     double last_ck = now_s(), last_u = last_ck;
     int consecutive_fail = 0;
     res->resumed_at = ui ? ui : vi;
@@ -267,7 +301,7 @@ extern "C" int kite_prp(uint64_t k, uint64_t n, uint32_t base, const kite_prp_op
         bool failed = false, round_fail = false;
         pend.clear();
         if (o.abort_flag && *o.abort_flag) {
-            if (o.checkpoint) ckpt_write(o.checkpoint, k, n, base, vi, kite_fft_len(s.ctx), vx.data(), vd.data(), nl);
+            if (o.checkpoint) write_verified();
             rc = KITE_PRP_ABORTED;
             goto out;
         }
@@ -334,6 +368,7 @@ extern "C" int kite_prp(uint64_t k, uint64_t n, uint32_t base, const kite_prp_op
                     goto out;
                 }
                 power_start(s, k, base, la.data());
+                if (!x0l.empty()) kite_set_limbs(s.ctx, s.x0, x0l.data());
                 res->roundoff_retries++;
                 continue;
             }
@@ -355,7 +390,7 @@ extern "C" int kite_prp(uint64_t k, uint64_t n, uint32_t base, const kite_prp_op
                 goto out;
             }
             if (o.checkpoint && (now_s() - last_ck > ck_secs || vi == full || stopping)) {
-                ckpt_write(o.checkpoint, k, n, base, vi, kite_fft_len(s.ctx), vx.data(), vd.data(), nl);
+                write_verified();
                 last_ck = now_s();
             }
             if (stopping) {

@@ -357,6 +357,8 @@ class JobResult(ctypes.Structure):
         ("confirm_runs", c_u32),
         ("start", c_dbl),
         ("finish", c_dbl),
+        ("resumed_at", c_u64),
+        ("resumed", c_u32),
     ]
 
 
@@ -375,6 +377,8 @@ class RunOpts(ctypes.Structure):
         ("inject_fault_at", c_u64),
         ("on_done", JOB_DONE_FN),
         ("arg", c_p),
+        ("checkpoint_dir", ctypes.c_char_p),
+        ("checkpoint_secs", c_dbl),
     ]
 
 
@@ -389,9 +393,9 @@ SIGNATURES.update({
 
 
 def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fault_job=0, fault_at=0,
-              max_roundoff=0.0, on_done=None):
+              max_roundoff=0.0, on_done=None, checkpoint_dir=None, checkpoint_secs=0.0):
     """Runs jobs [(kind, k, n, arrival), ...] (kind JOB_PRP or JOB_LLR; Fermat base 3) through
-    kite_batch_run. Returns (rc, [JobResult])."""
+    kite_batch_run, with checkpoint files in checkpoint_dir if given. Returns (rc, [JobResult])."""
     arr = (Job * len(jobs))(*[Job(k, n, kind, 3, 0, arrival) for kind, k, n, arrival in jobs])
     o = RunOpts()
     o.slots = slots
@@ -402,6 +406,9 @@ def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fa
     o.horizon = horizon
     o.inject_fault_job = fault_job
     o.inject_fault_at = fault_at
+    if checkpoint_dir is not None:
+        o.checkpoint_dir = os.fsencode(checkpoint_dir)
+        o.checkpoint_secs = checkpoint_secs
     keep = None
     if on_done is not None:
         keep = JOB_DONE_FN(lambda a, i, r: on_done(i, r.contents))

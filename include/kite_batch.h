@@ -110,6 +110,8 @@ typedef struct {
     double maxerr;
     uint32_t gerbicz_errors, roundoff_retries, confirm_runs;
     double start, finish;  // seconds after the start of kite_batch_run (finish: result known)
+    uint64_t resumed_at;   // iteration restored from a checkpoint (0: none, or an LLR run's start)
+    uint32_t resumed;      // 1 if the job started from a checkpoint
 } kite_job_result;
 
 typedef struct {
@@ -123,6 +125,14 @@ typedef struct {
     uint64_t inject_fault_at;   // (LLR: in its first run, so the vote has to outvote it)
     void (*on_done)(void *arg, size_t index, const kite_job_result *r);
     void *arg;
+    // Checkpoints (NULL: none). A job resumes from its file in this directory when it starts, and
+    // writes it after a verified block at most every checkpoint_secs (0 = 600) and for every job in
+    // flight when the horizon stops the run; the file is removed when the job finishes. Fermat
+    // jobs use kite_prp's KITECKP1 file "kite_<fp>.<base>.ckpt", LLR jobs kite_llr's KITELLR1
+    // file "kite_<fp>.llr.ckpt" (fp = N mod 3417905339), the names kite-prst uses, so a test
+    // stopped in one of them can be finished by the others.
+    const char *checkpoint_dir;
+    double checkpoint_secs;
 } kite_run_opts;
 
 // Device bytes a job of this kind holds at length L while in flight (the admission estimate).
