@@ -267,6 +267,9 @@ struct Engine {
     PassKernel kcfused = nullptr;   // square-mode fused column pass, if any
     unsigned long long *flags = nullptr;  // its per-chunk carry flags
     uint32_t seq = 0;                     // its launches so far
+    SlotDev *d_slot = nullptr;            // one-entry slot table read by the square graph
+    hipGraphExec_t sq_graph = nullptr;    // kSqGraphSteps squarings of the number in d_slot
+    bool sq_graph_failed = false;
     std::string name[3];  // kernel names of the three passes, for timing reports
     uint32_t H = 0;
 };
@@ -434,7 +437,8 @@ void engine_free(kite_ctx *ctx) {
     Engine *E = ctx->eng;
     if (!E) return;
     if (E->stream) hipStreamSynchronize(E->stream);
-    void *ps[] = {E->ta, E->tb, E->dcode, E->dig, E->cout, E->flags, E->yfreq, E->maxerr};
+    if (E->sq_graph) hipGraphExecDestroy(E->sq_graph);
+    void *ps[] = {E->ta, E->tb, E->dcode, E->dig, E->cout, E->flags, E->yfreq, E->maxerr, E->d_slot};
     for (void *p : ps)
         if (p) hipFree(p);
     if (E->stream) hipStreamDestroy(E->stream);
@@ -465,35 +469,94 @@ static void launch_pass(PassKernel fn, int grid, size_t lds, hipStream_t st, con
     hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st);
 }
 
+// The passes read their slot from `tbl` (a device table, step `step` of a multi-step launch
+// sequence) or, with tbl null, from the by-value copy `s`.
+static void launch_rows(Engine *E, const SlotDev &s, int mode, const SlotDev *tbl = nullptr, int step = 0) {
+    if (mode == kModeSquare && E->krows_sq)
+        launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, tbl, s, nullptr, mode, step, kRowsSqThreads);
+    else
+        launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, tbl, s, nullptr, mode, step);
+}
+
 static void launch_rows(Engine *E, double2 *x, const double2 *ysrc, int mode) {
     SlotDev s = E->slot;
     s.x = x;
     s.y_src = ysrc;
-    if (mode == kModeSquare && E->krows_sq)
-        launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, nullptr, s, nullptr, mode, 0, kRowsSqThreads);
-    else
-        launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, nullptr, s, nullptr, mode, 0);
+    launch_rows(E, s, mode);
+}
+
+static void launch_cols(Engine *E, const SlotDev &s, int mode, const SlotDev *tbl = nullptr, int step = 0) {
+    launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, tbl, s, nullptr, mode, step);
+    launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, tbl, s, nullptr, mode, step);
 }
 
 static void launch_cols(Engine *E, double2 *x, int mode) {
     SlotDev s = E->slot;
     s.x = x;
-    launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, mode, 0);
-    launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, mode, 0);
+    launch_cols(E, s, mode);
+}
+
+// One squaring's launches: rows, then the fused column pass (fused step `step` uses seq s.seq +
+// step + 1) or the two column passes.
+static void launch_square(Engine *E, const SlotDev &s, const SlotDev *tbl, int step) {
+    launch_rows(E, s, kModeSquare, tbl, step);
+    if (E->kcfused) launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, tbl, s, nullptr, kModeSquare, step);
+    else launch_cols(E, s, kModeSquare, tbl, step);
+}
+
+// Squarings of one context in runs of kSqGraphSteps are replayed from a captured HIP graph whose
+// kernels read the number, the launch counter and the step count from a one-entry device slot
+// table, written before each replay by a small kernel (its arguments are copied at launch, so no
+// host buffer has to outlive the call). Replays remove the host's per-launch cost, which bounds
+// a single test's squaring otherwise.
+constexpr uint32_t kSqGraphSteps = 32;
+
+__global__ void k_set_slot(SlotDev *dst, SlotDev v) {
+    static_assert(sizeof(SlotDev) % 8 == 0 && sizeof(SlotDev) / 8 <= 64, "slot copied as 64-bit words");
+    const uint32_t i = threadIdx.x;
+    if (i < sizeof(SlotDev) / 8) reinterpret_cast<uint64_t *>(dst)[i] = reinterpret_cast<const uint64_t *>(&v)[i];
+}
+
+static bool build_square_graph(Engine *E) {
+    if (E->sq_graph) return true;
+    if (E->sq_graph_failed) return false;
+    E->sq_graph_failed = true;  // until the capture below succeeds
+    if (!E->d_slot && hipMalloc(&E->d_slot, sizeof(SlotDev)) != hipSuccess) {
+        E->d_slot = nullptr;
+        (void)hipGetLastError();
+        return false;
+    }
+    hipGraph_t g = nullptr;
+    bool ok = hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        for (uint32_t st = 0; st < kSqGraphSteps; st++) launch_square(E, E->slot, E->d_slot, (int)st);
+        ok = hipStreamEndCapture(E->stream, &g) == hipSuccess && g &&
+             hipGraphInstantiate(&E->sq_graph, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) hipGraphDestroy(g);
+    }
+    (void)hipGetLastError();  // a failed capture falls back to direct launches
+    if (!ok) E->sq_graph = nullptr;
+    E->sq_graph_failed = !ok;
+    return ok;
 }
 
 void engine_square(kite_ctx *ctx, void *x, uint64_t count) {
     Engine *E = ctx->eng;
-    for (uint64_t i = 0; i < count; i++) {
-        launch_rows(E, (double2 *)x, nullptr, kModeSquare);
-        if (E->kcfused) {
-            SlotDev s = E->slot;
-            s.x = (double2 *)x;
-            s.seq = E->seq++;
-            launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, nullptr, s, nullptr, kModeSquare, 0);
-        } else {
-            launch_cols(E, (double2 *)x, kModeSquare);
+    SlotDev s = E->slot;
+    s.x = (double2 *)x;
+    if (count >= kSqGraphSteps && build_square_graph(E)) {
+        s.count = kSqGraphSteps;
+        for (; count >= kSqGraphSteps; count -= kSqGraphSteps) {
+            s.seq = E->seq;
+            hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, E->stream, E->d_slot, s);
+            hipGraphLaunch(E->sq_graph, E->stream);
+            E->seq += kSqGraphSteps;
         }
+        s.count = E->slot.count;
+    }
+    for (uint64_t i = 0; i < count; i++) {
+        s.seq = E->seq++;
+        launch_square(E, s, nullptr, 0);
     }
 }
 

@@ -156,14 +156,13 @@ def lib():
 
 
 def to_limbs(v, nl):
-    arr = (c_u64 * nl)()
-    for i in range(nl):
-        arr[i] = (v >> (64 * i)) & 0xFFFFFFFFFFFFFFFF
-    return arr
+    """The low 64*nl bits of v >= 0 as nl little-endian 64-bit limbs (linear in the size)."""
+    v &= (1 << (64 * nl)) - 1
+    return (c_u64 * nl).from_buffer_copy(v.to_bytes(8 * nl, "little"))
 
 
 def from_limbs(arr):
-    return sum(int(x) << (64 * i) for i, x in enumerate(arr))
+    return int.from_bytes(bytes(arr), "little")
 
 
 class Ctx:

@@ -88,7 +88,5 @@ def random_residue_limbs(N, seed):
     """A residue uniform mod N from xorshift64 limbs (as the reference's test_gmp.c draws them)."""
     g = XorShift64(seed)
     nl = (N.bit_length() + 63) // 64 + 1
-    v = 0
-    for i in range(nl):
-        v |= g.next() << (64 * i)
-    return v % N
+    limbs = b"".join(g.next().to_bytes(8, "little") for _ in range(nl))  # limb i = i-th draw
+    return int.from_bytes(limbs, "little") % N

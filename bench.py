@@ -2,6 +2,7 @@
 """Benchmark of the MI355X squaring engine on the Fermat workload at n ~ 660K (FFT ~49152).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--cpu-secs S] [--no-cpu]
+  python bench.py --workload {batch_mixed,large_fft,record,plain_small} [...]
   python bench.py --serve [--serve-count C] [--serve-rate R] [--serve-horizon S] ...
 
 A step is one squaring of every residue in the batch (B Fermat base-3 tests with k odd in
@@ -32,6 +33,21 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# the batch workloads (ahnum_amd/workloads.py): metric, description, test count (None: --batch)
+WORKLOADS = {
+    "single_large": ("tests/hour at n~660K (Fermat base 3, FFT 49152, Gerbicz-checkable residues), batched",
+                     "single_large batch: B Fermat base-3 tests, k odd in [1001,9999], n in [650000,670000]",
+                     None, 2),
+    "batch_mixed": ("tests/hour, 256 Fermat base-3 tests at n 300K-700K (mixed FFT lengths), batched",
+                    "batch_mixed: 256 Fermat base-3 tests, k odd in [3,9999], n in [300000,700000]", 256, 3),
+    "large_fft": ("tests/hour, 32 Fermat base-3 tests at n 3M-6M (FFT 160K-442K), batched",
+                  "large_fft: 32 Fermat base-3 tests, k odd in [3,999], n in [3000000,6000000]", 32, 4),
+    "record": ("tests/hour of a record-size Fermat test at n 24M-25M (FFT ~1.6M)",
+               "record: 1 Fermat base-3 test, k odd < 100, n in [24000000,25000000]", 1, 5),
+    "plain_small": ("tests/hour, 16 Fermat base-3 tests at n 60K-70K (FFT ~4K), batched",
+                    "plain_small: 16 Fermat base-3 tests, k odd in [1001,9999], n in [60000,70000]", 16, 1),
+}
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA counted as two)
@@ -140,7 +156,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--batch", type=int, default=384)
+    ap.add_argument("--batch", type=int, default=384, help="tests of the single_large workload")
+    ap.add_argument("--workload", default="single_large", choices=sorted(WORKLOADS),
+                    help="batch workload (single_large: the n~660K batch the metric is quoted on)")
     ap.add_argument("--plain-steps", type=int, default=3000)
     ap.add_argument("--cpu-secs", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -181,7 +199,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    tests = workloads.single_large(a.batch)
+    metric, described, count, seed = WORKLOADS[a.workload]
+    if count is None:
+        tests = workloads.single_large(a.batch)
+    elif a.workload == "record":
+        tests = workloads.record()
+    else:
+        tests = getattr(workloads, a.workload)(count)
 
     # ---- plain path: one test at a time through kite.h ----
     k0, n0 = tests[0]
@@ -241,13 +265,13 @@ def main():
     avg_s = kms / 1e3 / klaunch
     alg_bytes = kbytes / klaunch
     achieved = alg_bytes / avg_s / 1e9
-    traffic = load_traffic(kname)
+    traffic = load_traffic(kname) if a.workload == "single_large" else None
     step_bytes = sum(32 * L for L in Ls)
     step_flops = sum(5 * L * math.log2(L / 2) + 20 * L for L in Ls)
 
     out = {
-        "metric": "tests/hour at n~660K (Fermat base 3, FFT 49152, Gerbicz-checkable residues), batched",
-        "value": round(tph, 1),
+        "metric": metric,
+        "value": round(tph, 3 if tph < 100 else 1),
         "unit": "tests/hour",
         "n_gpus": 1,
         "steps": a.steps,
@@ -259,11 +283,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded random residues mod k*2^n-1)",
         "config": {
-            "workload": "single_large batch: B Fermat base-3 tests, k odd in [1001,9999], n in [650000,670000]",
-            "batch": a.batch,
+            "workload": described,
+            "batch": len(tests),
             "fft_lengths": sorted(set(Ls)),
             "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams",
-            "seed": 2,
+            "seed": seed,
         },
         "roofline": {
             "bound": "hbm",
@@ -297,7 +321,7 @@ def main():
             "fp64_achieved_tflops": round(step_flops * a.steps / wall / 1e12, 2),
             "fp64_frac": round(step_flops * a.steps / wall / 1e12 / FP64_PEAK_TFLOPS, 4),
         },
-        "batched_us_per_test_squaring": round(wall * 1e6 / a.steps / a.batch, 4),
+        "batched_us_per_test_squaring": round(wall * 1e6 / a.steps / len(tests), 4),
         "batched_vs_plain": round(tph / plain_tph, 2),
         "maxerr": round(max(errs.values()), 4),
         "gpu_ms_per_timed_step": round(sum(ms) / max(launches[0], 1), 4),

@@ -71,11 +71,12 @@ def lib():
 
 
 def to_limbs(v, nl):
-    return (u64 * nl)(*[(v >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(nl)])
+    v &= (1 << (64 * nl)) - 1
+    return (u64 * nl).from_buffer_copy(v.to_bytes(8 * nl, "little"))
 
 
 def from_limbs(a):
-    return sum(int(x) << (64 * i) for i, x in enumerate(a))
+    return int.from_bytes(bytes(a), "little")
 
 
 class OCtx:
