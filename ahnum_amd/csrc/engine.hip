@@ -540,10 +540,11 @@ static bool build_square_graph(Engine *E) {
     return ok;
 }
 
-void engine_square(kite_ctx *ctx, void *x, uint64_t count) {
+void engine_square(kite_ctx *ctx, void *x, uint64_t count, int32_t xadd) {
     Engine *E = ctx->eng;
     SlotDev s = E->slot;
     s.x = (double2 *)x;
+    s.xadd = xadd;
     if (count >= kSqGraphSteps && build_square_graph(E)) {
         s.count = kSqGraphSteps;
         for (; count >= kSqGraphSteps; count -= kSqGraphSteps) {

@@ -73,7 +73,7 @@ void engine_num_free(kite_ctx *ctx, void *p);
 void engine_copy(kite_ctx *ctx, void *dst, const void *src);
 void engine_from_digits(kite_ctx *ctx, void *x, const int64_t *dig);
 void engine_to_digits(kite_ctx *ctx, const void *x, int64_t *dig);
-void engine_square(kite_ctx *ctx, void *x, uint64_t count);
+void engine_square(kite_ctx *ctx, void *x, uint64_t count, int32_t xadd = 0);  // x -> x^2 + xadd, count times
 void engine_mul(kite_ctx *ctx, void *x, const void *y);
 void engine_addsub(kite_ctx *ctx, void *x, const void *y, int sign);
 double engine_maxerr(kite_ctx *ctx);
@@ -85,6 +85,9 @@ const char *engine_last_error();
 // x = V_k(P, -1) by the Lucas ladder (cP holds P, c2 holds 2; v1, t are scratch).
 void llr_lucas_vk(kite_ctx *ctx, uint64_t k, kite_num *x, kite_num *v1, kite_num *cP, kite_num *c2, kite_num *t);
 uint64_t n_mod(uint64_t k, uint64_t n, uint64_t m);  // (k 2^n - 1) mod m
+// x -> x^2 + xadd, count times (|xadd| < 2^20): the addend enters in the carry pass's top fold,
+// so an LLR step costs one squaring (api.cpp)
+void square_add(kite_ctx *ctx, kite_num *x, uint64_t count, int32_t xadd);
 
 // ---- checkpoint files (prp.cpp, llr.cpp) -----------------------------------------------------
 inline size_t limbs_for_n(uint64_t n) { return (n + 64 + 63) / 64; }  // limbs of a residue, k < 2^64

@@ -178,11 +178,13 @@ int llr_run(uint64_t k, uint64_t n, uint32_t P, const kite_llr_opts *o, uint32_t
         while (!round_fail && i < total) {
             kite_reset_roundoff(ctx);
             const uint64_t stop = i + B < total ? i + B : total;
-            for (; i < stop; i++) {
-                kite_square(ctx, x);
-                if (i == 0) kite_add(ctx, x, c2);  // V_2k = V_k^2 + 2 (k odd)
-                else kite_sub(ctx, x, c2);
-                if (fault_due && i + 1 == o->inject_fault_at) {
+            while (i < stop) {  // V_2k = V_k^2 + 2 (k odd), then x -> x^2 - 2, both inside the squaring
+                uint64_t end = stop;
+                if (fault_due && o->inject_fault_at > i && o->inject_fault_at < end) end = o->inject_fault_at;
+                if (i == 0) end = 1;
+                square_add(ctx, x, end - i, i == 0 ? 2 : -2);
+                i = end;
+                if (fault_due && i == o->inject_fault_at) {
                     kite_debug_corrupt(ctx, x);
                     fault_due = false;
                 }
