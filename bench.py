@@ -159,7 +159,7 @@ def main():
     ap.add_argument("--batch", type=int, default=384, help="tests of the single_large workload")
     ap.add_argument("--workload", default="single_large", choices=sorted(WORKLOADS),
                     help="batch workload (single_large: the n~660K batch the metric is quoted on)")
-    ap.add_argument("--plain-steps", type=int, default=3000)
+    ap.add_argument("--plain-steps", type=int, default=10000, help="squarings of the one-test microbench")
     ap.add_argument("--cpu-secs", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-every", type=int, default=10,

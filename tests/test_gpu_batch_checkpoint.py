@@ -109,3 +109,21 @@ def test_kite_prst_finishes_what_a_batch_stopped(tmp_path):
         assert m and float(m.group(1)) > 0, out[-400:]
         assert expect in out, out[-400:]
     assert not [f for f in os.listdir(tmp_path) if f.endswith(".ckpt")]
+
+
+def test_record_size_gerbicz_checked_squarings(tmp_path):
+    """The record-size configuration: 91*2^24767982-1 (workloads.record, FFT 1835008) runs for a
+    20 s horizon with a Gerbicz check every 10 blocks of 1000; at least 10^5 squarings are verified
+    (the checkpoint's iteration) with no check failing and round-off below 0.4."""
+    from ahnum_amd import workloads
+    (k, n), = workloads.record()
+    say("record", k, n)
+    rc, out = kite.batch_run([(kite.JOB_PRP, k, n, 0.0)], block=1000, blocks=10, horizon=20.0,
+                             checkpoint_dir=str(tmp_path))
+    r = out[0]
+    say("status", r.status, "fft", r.fft_len, "maxerr", r.maxerr, "gerbicz errors", r.gerbicz_errors)
+    assert r.status == kite.JOB_PENDING and r.fft_len == 1835008
+    assert r.gerbicz_errors == 0 and r.roundoff_retries == 0 and 0 < r.maxerr < 0.4
+    magic, key, it, L, x, d = read_fermat(fermat_file(tmp_path, k, n))
+    say("verified iterations", it)
+    assert magic == b"KITECKP1" and key == (k, n, 3) and L == 1835008 and it >= 100000 and it % 10000 == 0
