@@ -127,3 +127,29 @@ def test_record_size_gerbicz_checked_squarings(tmp_path):
     magic, key, it, L, x, d = read_fermat(fermat_file(tmp_path, k, n))
     say("verified iterations", it)
     assert magic == b"KITECKP1" and key == (k, n, 3) and L == 1835008 and it >= 100000 and it % 10000 == 0
+
+
+@pytest.mark.skipif(not os.environ.get("KITE_LONG_TESTS"), reason="about 6 minutes of GPU: set KITE_LONG_TESTS=1")
+def test_large_fft_reference_residues_in_checkpointed_chunks(tmp_path, golden):
+    """The reference's tests/large_fft_cases.csv: 7 numbers at n 2.3M-5.5M (FFT 160K-442K) whose
+    RES64s come from PRST and kite, in one batch run as 60 s horizon chunks that resume from their
+    checkpoint files (Gerbicz check every 20 blocks of 1000). Every RES64 must match."""
+    rows = golden("large_fft_cases.csv")
+    pending = {i: (int(k), int(n)) for i, (k, n, *_rest) in enumerate(rows)}
+    results = {}
+    for chunk in range(15):
+        if not pending:
+            break
+        idx = sorted(pending)
+        rc, out = kite.batch_run([(kite.JOB_PRP, *pending[i], 0.0) for i in idx], block=1000, blocks=20,
+                                 horizon=60.0, checkpoint_dir=str(tmp_path))
+        for i, r in zip(idx, out):
+            if r.status != kite.JOB_PENDING:
+                results[i] = r
+                del pending[i]
+        say("chunk", chunk, "finished", sorted(results), "resumed", [r.resumed_at for r in out])
+    assert not pending
+    for i, (k, n, want, ref_fft, _src) in enumerate(rows):
+        r = results[i]
+        say(k, n, r.status, "%016X" % r.res64, "fft", r.fft_len, "(reference", ref_fft + ")", "maxerr", r.maxerr)
+        assert r.status == 0 and "%016X" % r.res64 == want and r.maxerr < 0.4, (k, n)
