@@ -550,7 +550,15 @@ void engine_square(kite_ctx *ctx, void *x, uint64_t count, int32_t xadd) {
         for (; count >= kSqGraphSteps; count -= kSqGraphSteps) {
             s.seq = E->seq;
             hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, E->stream, E->d_slot, s);
-            hipGraphLaunch(E->sq_graph, E->stream);
+            if (hipGraphLaunch(E->sq_graph, E->stream) != hipSuccess) {
+                // this run and the rest go through direct launches; the graph is not tried again
+                (void)hipGetLastError();
+                hipStreamSynchronize(E->stream);  // earlier replays finish before the graph goes
+                hipGraphExecDestroy(E->sq_graph);
+                E->sq_graph = nullptr;
+                E->sq_graph_failed = true;
+                break;
+            }
             E->seq += kSqGraphSteps;
         }
         s.count = E->slot.count;
