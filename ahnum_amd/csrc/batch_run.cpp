@@ -23,6 +23,9 @@
 #include <thread>
 #include <vector>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "kite_batch.h"
 #include "kite_internal.h"
 
@@ -361,7 +364,18 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
     R.B = R.o.block ? R.o.block : 1000;
     R.M = R.o.gerbicz_blocks ? R.o.gerbicz_blocks : std::max<uint32_t>(R.B / 4, 8);
     R.maxround = R.o.max_roundoff > 0 ? R.o.max_roundoff : 0.40;
-    if (R.o.checkpoint_dir && R.o.checkpoint_dir[0]) R.dir = R.o.checkpoint_dir;
+    if (R.o.checkpoint_dir && R.o.checkpoint_dir[0]) {
+        R.dir = R.o.checkpoint_dir;
+        struct stat st;
+        if (stat(R.dir.c_str(), &st) != 0 || !S_ISDIR(st.st_mode) || access(R.dir.c_str(), W_OK | X_OK) != 0) {
+            // not a writable directory: refuse before any work
+            for (size_t i = 0; i < njobs; i++) {
+                res[i] = kite_job_result{};
+                res[i].status = -1;
+            }
+            return -1;
+        }
+    }
     R.ck_secs = R.o.checkpoint_secs > 0 ? R.o.checkpoint_secs : 600;
     const size_t max_slots = R.o.slots ? R.o.slots : (size_t)-1;
     R.b = kite_batch_new();

@@ -125,7 +125,8 @@ typedef struct {
     uint64_t inject_fault_at;   // (LLR: in its first run, so the vote has to outvote it)
     void (*on_done)(void *arg, size_t index, const kite_job_result *r);
     void *arg;
-    // Checkpoints (NULL: none). A job resumes from its file in this directory when it starts, and
+    // Checkpoints (NULL: none). The directory must exist and be writable, else kite_batch_run
+    // returns -1 at once with every status -1. A job resumes from its file there when it starts, and
     // writes it after a verified block at most every checkpoint_secs (0 = 600) and for every job in
     // flight when the horizon stops the run; the file is removed when the job finishes. Fermat
     // jobs use kite_prp's KITECKP1 file "kite_<fp>.<base>.ckpt", LLR jobs kite_llr's KITELLR1

@@ -93,3 +93,15 @@ def test_ctx_rejects_cpu_backends_and_bad_input():
     o = kite.KiteOpts(0, 0, 0, 0)
     assert not lib.kite_ctx_new(9358, 20000, -1, ctypes.byref(o))  # even k
     assert not lib.kite_ctx_new(9357, 20000, 1, ctypes.byref(o))   # c != -1
+
+
+def test_batch_run_refuses_an_unusable_checkpoint_dir(tmp_path):
+    """kite_batch_run with a checkpoint directory that is missing, or a file, returns -1 at once
+    with every status -1, before it touches the GPU (include/kite_batch.h)."""
+    from ahnum_amd import kite
+    f = tmp_path / "a_file"
+    f.write_text("x")
+    for d in (tmp_path / "missing", f):
+        rc, out = kite.batch_run([(kite.JOB_PRP, 9357, 20000, 0.0), (kite.JOB_LLR, 3, 26459, 0.0)],
+                                 checkpoint_dir=str(d))
+        assert rc == -1 and [r.status for r in out] == [-1, -1]
