@@ -98,7 +98,6 @@ def test_ctx_rejects_cpu_backends_and_bad_input():
 def test_batch_run_refuses_an_unusable_checkpoint_dir(tmp_path):
     """kite_batch_run with a checkpoint directory that is missing, or a file, returns -1 at once
     with every status -1, before it touches the GPU (include/kite_batch.h)."""
-    from ahnum_amd import kite
     f = tmp_path / "a_file"
     f.write_text("x")
     for d in (tmp_path / "missing", f):
