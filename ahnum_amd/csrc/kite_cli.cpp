@@ -5,12 +5,13 @@
 //                                 [--uckpt FILE] [--uckpt-secs S] [--block B] [--fault ITER]
 //   kite llr  K N [--expect HEX|prime] [--ckpt FILE]
 //   kite time K N [--iters N]
-//   kite throughput K N [--configs 10x1,5x2,...] [--secs S]
+//   kite throughput K N [--configs 10x1,5x2,...] [--secs S] [--batch 1]
 //   kite tune LMIN LMAX [--out FILE]
 // Common options: --backend hip --fft L --threads T (ignored) --order 0..3 (shape variant)
 //
 // `throughput` runs T concurrent tests, each with its own context and HIP stream on the GPU,
-// from T host threads (the second number of each config, threads per test, is ignored).
+// from T host threads (the second number of each config, threads per test, is ignored); with
+// --batch 1 the T tests form one batch instead (include/kite_batch.h: shared launches per pass).
 #include <atomic>
 #include <cinttypes>
 #include <cstdio>
@@ -21,6 +22,7 @@
 #include <vector>
 
 #include "kite.h"
+#include "kite_batch.h"
 
 static double now_s() {
     timespec ts;
@@ -64,7 +66,7 @@ static void usage(const char *p) {
             "usage: %s prp K N | \"K*2^N-1\" [--expect HEX] [--base B] [--ckpt FILE]\n"
             "       %s llr K N | \"K*2^N-1\" [--expect HEX|prime] [--ckpt FILE]\n"
             "       %s time K N [--iters N]\n"
-            "       %s throughput K N [--configs 10x1,5x2,...] [--secs S]\n"
+            "       %s throughput K N [--configs 10x1,5x2,...] [--secs S] [--batch 1]\n"
             "       %s tune LMIN LMAX [--out FILE]\n"
             "options: --backend hip --fft L --threads T\n"
             "         --order 0..3 (shape variant, default from the tuning file)\n",
@@ -112,6 +114,34 @@ static double throughput(uint64_t k, uint64_t n, kite_opts o, int tests, double 
     return tot;
 }
 
+// T tests of the number as one batch for `secs`, in square calls of 1000 steps (a Gerbicz block,
+// replayed from the batch's captured graph); returns squarings/s summed over tests, 0 on failure.
+static double throughput_batched(uint64_t k, uint64_t n, uint32_t L, int tests, double secs) {
+    kite_batch *b = kite_batch_new();
+    if (!b) return 0;
+    for (int t = 0; t < tests; t++) {
+        const int s = kite_batch_add(b, k, n, L, 1);
+        if (s < 0) {
+            kite_batch_free(b);
+            return 0;
+        }
+        kite_set_u64(kite_batch_ctx(b, s), kite_batch_num(b, s, 0), 3 + (uint64_t)t);
+    }
+    const int cap = kite_batch_capacity(b);
+    std::vector<int32_t> which(cap, 0);
+    std::vector<uint32_t> cnt(cap, 1000);
+    bool ok = kite_batch_square(b, which.data(), cnt.data()) == 0 && kite_batch_sync(b) == 0;  // warm-up
+    uint64_t it = 0;
+    const double t0 = now_s();
+    while (ok && now_s() - t0 < secs) {
+        ok = kite_batch_square(b, which.data(), cnt.data()) == 0 && kite_batch_sync(b) == 0;
+        it += 1000;
+    }
+    const double dt = now_s() - t0;
+    kite_batch_free(b);
+    return ok ? (double)it * tests / dt : 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 3) {
         usage(argv[0]);
@@ -130,6 +160,7 @@ int main(int argc, char **argv) {
     uint64_t iters = 1000;
     uint32_t base = 3;
     double secs = 3;
+    bool batched = false;
     for (int i = 2 + used; i + 1 < argc; i += 2) {
         const char *a = argv[i], *v = argv[i + 1];
         if (!strcmp(a, "--backend")) o.backend = parse_backend(v);
@@ -147,6 +178,7 @@ int main(int argc, char **argv) {
         else if (!strcmp(a, "--fault")) po.inject_fault_at = strtoull(v, nullptr, 10);
         else if (!strcmp(a, "--configs")) cfg = v;
         else if (!strcmp(a, "--secs")) secs = atof(v);
+        else if (!strcmp(a, "--batch")) batched = atoi(v) != 0;
         else if (!strcmp(a, "--out")) out = v;
         else {
             fprintf(stderr, "unknown option %s\n", a);
@@ -242,8 +274,8 @@ int main(int argc, char **argv) {
         char bestc[32] = "";
         for (char *tok = strtok(buf, ","); tok; tok = strtok(nullptr, ",")) {
             int T = 1, S = 1;
-            if (sscanf(tok, "%dx%d", &T, &S) < 1 || T < 1 || T > 256) continue;
-            const double r = throughput(k, n, o, T, secs);
+            if (sscanf(tok, "%dx%d", &T, &S) < 1 || T < 1 || T > (batched ? 4096 : 256)) continue;
+            const double r = batched ? throughput_batched(k, n, o.fft_len, T, secs) : throughput(k, n, o, T, secs);
             printf("%-6s %10.0f sq/s  %8.1f tests/hour  (%.1f us/iter per test)\n", tok, r, r * 3600.0 / n, 1e6 * T / r);
             fflush(stdout);
             if (r > best) {
