@@ -264,6 +264,9 @@ struct Engine {
     size_t lds_rows = 0, lds_cols = 0, lds_fused = 0;
     PassKernel krows = nullptr, kcinv = nullptr, kcfwd = nullptr;
     PassKernel krows_sq = nullptr;  // square-mode rows with registers, if any
+    PassKernel krows1 = nullptr;    // one row pair per work-group, for this context's own squarings
+    int nrowjobs1 = 0;
+    size_t lds_rows1 = 0;
     PassKernel kcfused = nullptr;   // square-mode fused column pass, if any
     unsigned long long *flags = nullptr;  // its per-chunk carry flags
     uint32_t seq = 0;                     // its launches so far
@@ -418,6 +421,10 @@ int engine_init(kite_ctx *ctx) {
         E->name[2] = ff ? "k_cols_fwd_r<" + std::to_string(sh.R) + ">" : std::string("k_cols_fwd_g");
         E->nrowjobs = (npairs + ppw - 1) / ppw;
         E->ncoljobs = sh.NG;
+        if (fr && ppw > 1 && (E->krows1 = fast_rows1(sh.C))) {
+            E->nrowjobs1 = npairs;
+            E->lds_rows1 = (size_t)16 * (2 * sh.C + ((2 * sh.C) >> kRowPad) + (sh.C <= (int)kTwLdsMaxC ? sh.C : 0));
+        }
         const int sub = sh.G < kSub ? sh.G : kSub;
         if (fr)  // single buffer of 2*ppw rows, then the row twiddles
             E->lds_rows = (size_t)16 * (2 * ppw * sh.C + ((2 * ppw * sh.C) >> kRowPad) +
@@ -496,10 +503,12 @@ static void launch_cols(Engine *E, double2 *x, int mode) {
     launch_cols(E, s, mode);
 }
 
-// One squaring's launches: rows, then the fused column pass (fused step `step` uses seq s.seq +
-// step + 1) or the two column passes.
+// One squaring of this context's number alone: rows (one pair per work-group where that kernel
+// exists, since a single test leaves most CUs idle and latency decides), then the fused column
+// pass (fused step `step` uses seq s.seq + step + 1) or the two column passes.
 static void launch_square(Engine *E, const SlotDev &s, const SlotDev *tbl, int step) {
-    launch_rows(E, s, kModeSquare, tbl, step);
+    if (E->krows1) launch_pass(E->krows1, E->nrowjobs1, E->lds_rows1, E->stream, tbl, s, nullptr, kModeSquare, step);
+    else launch_rows(E, s, kModeSquare, tbl, step);
     if (E->kcfused) launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, tbl, s, nullptr, kModeSquare, step);
     else launch_cols(E, s, kModeSquare, tbl, step);
 }

@@ -810,7 +810,9 @@ template <int N, int HOW> struct LastStage {
 };
 
 // Rows: 2*PPW rows of C in LDS (row slot 2i+h = side h of local pair i), twiddles w_C after them.
-template <int C>
+// PPW_T pairs per work-group: ppw_for_C(C) fills every stage's threads; 1 spreads one test's
+// rows over more work-groups (lower latency when a single test runs alone).
+template <int C, int PPW_T = ppw_for_C(C)>
 __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
                                           int step) {
     extern __shared__ double2 lds[];
@@ -818,7 +820,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
     const SlotDev &s = pick_slot(slots, one, jb.x);
     if (step >= (int)s.count) return;
     const ShapeDev &sh = *s.shp;
-    constexpr int PPW = ppw_for_C(C), NROW = 2 * PPW, NE = NROW * C;
+    constexpr int PPW = PPW_T, NROW = 2 * PPW, NE = NROW * C;
     constexpr bool TWL = C <= kTwLdsMaxC;
     constexpr int PD = kRowPad;  // row buffer index i lives at i + i/8
     const int R = sh.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;

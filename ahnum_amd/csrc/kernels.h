@@ -16,6 +16,8 @@ PassKernel generic_cols_inv();
 PassKernel generic_cols_fwd();
 // nullptr when no compile-time kernel exists for that size
 PassKernel fast_rows(int C);
+// the same with one row pair per work-group (C < 1024; nullptr otherwise)
+PassKernel fast_rows1(int C);
 // square-mode row kernel with the rows held in registers (128 threads per work-group, four row
 // pairs like the LDS row kernel, static LDS only); nullptr for row lengths without one
 PassKernel fast_rows_sq(int C);

@@ -10,6 +10,11 @@ template <int C>
 __global__ void __launch_bounds__(kNT) k_rows_c(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
     rows_fast<C>(slots, one, jobs, mode, step);
 }
+// One row pair per work-group: a single test's row pass on four times as many work-groups.
+template <int C>
+__global__ void __launch_bounds__(kNT) k_rows_c1(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+    rows_fast<C, 1>(slots, one, jobs, mode, step);
+}
 __global__ void __launch_bounds__(128, 3) k_rows_sq256(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
     (void)mode;
     rows256_sq(slots, one, jobs, step);
@@ -108,6 +113,16 @@ PassKernel fast_rows(int C) {
     case 1024: return (PassKernel)k_rows_c<1024>;
     case 2048: return (PassKernel)k_rows_c<2048>;
     default: return nullptr;
+    }
+}
+
+PassKernel fast_rows1(int C) {
+    switch (C) {
+    case 64: return (PassKernel)k_rows_c1<64>;
+    case 128: return (PassKernel)k_rows_c1<128>;
+    case 256: return (PassKernel)k_rows_c1<256>;
+    case 512: return (PassKernel)k_rows_c1<512>;
+    default: return nullptr;  // longer rows already take one pair per work-group
     }
 }
 
@@ -332,8 +347,10 @@ void all_kernels(PassKernel *out, int *count, int cap) {
     for (int i = 0; i < 3 && n < cap; i++) out[n++] = g[i];
     const int Cs[] = {64, 128, 256, 512, 1024, 2048};
     const int Rs[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96, 112, 128, 160, 192, 224, 256, 320, 384, 448, 512, 640, 768};
-    for (int c : Cs)
+    for (int c : Cs) {
         if (n < cap) out[n++] = fast_rows(c);
+        if (fast_rows1(c) && n < cap) out[n++] = fast_rows1(c);
+    }
     for (int r : Rs) {
         if (n < cap) out[n++] = fast_cols_inv(r);
         if (n < cap) out[n++] = fast_cols_fwd(r);
