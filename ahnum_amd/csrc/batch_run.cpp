@@ -8,7 +8,8 @@
 // then n - 1 doublings x -> x^2 - 2 (the first + 2) done by the squaring kernels themselves
 // (kite_batch_set_xadd), a round-off check after every block with a device copy of the last
 // verified state, and a composite answer confirmed at the next FFT length (a third run breaks a
-// tie). Jobs start once they have arrived and the slot and HBM limits allow; a finished job's
+// tie). Jobs start once they have arrived and the slot and HBM limits allow, by arrival or least
+// work first; a finished job's
 // slot goes to the next one at once (continuous batching). With a checkpoint directory a job
 // resumes from its verified state on file and writes it back as it goes (src/prp.c:156-212 and
 // src/llr.c:92-146 for the two file formats).
@@ -406,12 +407,23 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
     auto admit = [&]() {
         const double now = now_s() - R.t0;
         while (!waiting.empty() && active.size() < max_slots) {
-            Job &t = R.all[waiting.front()];
+            auto pick = waiting.begin();
+            if (R.o.order == KITE_ORDER_LEAST_WORK) {  // the arrived job with the least work: n^2
+                double best = 0;                        // (n squarings at a length proportional to n)
+                for (auto it = waiting.begin(); it != waiting.end() && R.all[*it].arrival <= now; ++it) {
+                    const double w = (double)R.all[*it].n * (double)R.all[*it].n;
+                    if (it == waiting.begin() || w < best) {
+                        best = w;
+                        pick = it;
+                    }
+                }
+            }
+            Job &t = R.all[*pick];
             if (t.arrival > now) break;
             const uint32_t Lest = t.L ? t.L : kite_choose_fft_len(t.k, t.n);
             const uint64_t need = kite_job_bytes(t.kind, Lest);
             if (R.o.hbm_budget && !active.empty() && R.bytes_in_flight + need > R.o.hbm_budget) break;
-            waiting.pop_front();
+            waiting.erase(pick);
             t.started = true;
             t.res.start = now;
             t.bytes = need;

@@ -335,6 +335,7 @@ SIGNATURES.update({
 
 
 JOB_PRP, JOB_LLR, JOB_PENDING = 0, 1, 1
+ORDER_ARRIVAL, ORDER_LEAST_WORK = 0, 1
 
 
 class Job(ctypes.Structure):
@@ -378,6 +379,7 @@ class RunOpts(ctypes.Structure):
         ("arg", c_p),
         ("checkpoint_dir", ctypes.c_char_p),
         ("checkpoint_secs", c_dbl),
+        ("order", c_u32),
     ]
 
 
@@ -392,7 +394,7 @@ SIGNATURES.update({
 
 
 def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fault_job=0, fault_at=0,
-              max_roundoff=0.0, on_done=None, checkpoint_dir=None, checkpoint_secs=0.0):
+              max_roundoff=0.0, on_done=None, checkpoint_dir=None, checkpoint_secs=0.0, order=ORDER_ARRIVAL):
     """Runs jobs [(kind, k, n, arrival), ...] (kind JOB_PRP or JOB_LLR; Fermat base 3) through
     kite_batch_run, with checkpoint files in checkpoint_dir if given. Returns (rc, [JobResult])."""
     arr = (Job * len(jobs))(*[Job(k, n, kind, 3, 0, arrival) for kind, k, n, arrival in jobs])
@@ -405,6 +407,7 @@ def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fa
     o.horizon = horizon
     o.inject_fault_job = fault_job
     o.inject_fault_at = fault_at
+    o.order = order
     if checkpoint_dir is not None:
         o.checkpoint_dir = os.fsencode(checkpoint_dir)
         o.checkpoint_secs = checkpoint_secs

@@ -116,8 +116,12 @@ def serve(a):
         print(f"done {i} status {r.status} n {jobs[i][2]} latency {r.finish - jobs[i][3]:.2f}s", file=sys.stderr,
               flush=True)
 
-    rc, out = kite.batch_run(jobs, hbm_budget=int(a.serve_hbm_gb * 2**30), block=1000, horizon=a.serve_horizon,
-                             on_done=progress)
+    if a.serve_checkpoint_dir:
+        os.makedirs(a.serve_checkpoint_dir, exist_ok=True)
+    order = kite.ORDER_LEAST_WORK if a.serve_order == "least_work" else kite.ORDER_ARRIVAL
+    rc, out = kite.batch_run(jobs, slots=a.serve_slots, hbm_budget=int(a.serve_hbm_gb * 2**30), block=1000,
+                             horizon=a.serve_horizon, on_done=progress, checkpoint_dir=a.serve_checkpoint_dir or None,
+                             checkpoint_secs=a.serve_checkpoint_secs, order=order)
     wall = time.perf_counter() - t0
     done = [(j, r) for j, r in zip(jobs, out) if r.status == 0]
     lat = sorted(r.finish - j[3] for j, r in done)
@@ -136,7 +140,8 @@ def serve(a):
         "data": "synthetic request stream (seeded)",
         "config": {"workload": "serve_stream", "requests": a.serve_count, "rate_per_s": a.serve_rate,
                    "n_range": [a.serve_n_lo, a.serve_n_hi], "horizon_s": a.serve_horizon,
-                   "hbm_budget_gib": a.serve_hbm_gb, "block": 1000},
+                   "hbm_budget_gib": a.serve_hbm_gb, "block": 1000, "slots": a.serve_slots,
+                   "order": a.serve_order},
         "completed": len(done),
         "completed_fermat": sum(1 for j, _ in done if j[0] == kite.JOB_PRP),
         "completed_llr": sum(1 for j, _ in done if j[0] == kite.JOB_LLR),
@@ -148,6 +153,10 @@ def serve(a):
         "wall_s": round(wall, 2),
         "maxerr": round(max((r.maxerr for _, r in done), default=0.0), 4),
     }
+    if a.serve_checkpoint_dir:  # restartable serving: the stopped jobs' states are on disk
+        out_line["checkpoint_dir"] = a.serve_checkpoint_dir
+        out_line["resumed"] = sum(1 for r in out if r.resumed)
+        out_line["checkpoint_files"] = sum(1 for f in os.listdir(a.serve_checkpoint_dir) if f.endswith(".ckpt"))
     print(json.dumps(out_line), flush=True)
 
 
@@ -177,6 +186,12 @@ def main():
     ap.add_argument("--serve-n-hi", type=int, default=5000000)
     ap.add_argument("--serve-horizon", type=float, default=120.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=0, help="jobs in flight at most (0 = HBM budget only)")
+    ap.add_argument("--serve-order", default="arrival", choices=["arrival", "least_work"],
+                    help="admission order among arrived requests")
+    ap.add_argument("--serve-checkpoint-dir", default="",
+                    help="jobs resume from and write checkpoints here; the horizon saves every job in flight")
+    ap.add_argument("--serve-checkpoint-secs", type=float, default=0.0, help="checkpoint interval (0 = 600 s)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (independent tests shard "

@@ -91,6 +91,7 @@ int kite_batch_prp(const kite_batch_test *tests, size_t ntests, const kite_batch
 
 enum { KITE_JOB_PRP = 0, KITE_JOB_LLR = 1 };
 enum { KITE_JOB_PENDING = 1 };  // status of a job the horizon stopped before it finished
+enum { KITE_ORDER_ARRIVAL = 0, KITE_ORDER_LEAST_WORK = 1 };  // kite_run_opts.order
 
 typedef struct {
     uint64_t k, n;
@@ -134,13 +135,17 @@ typedef struct {
     // stopped in one of them can be finished by the others.
     const char *checkpoint_dir;
     double checkpoint_secs;
+    // Admission order among the jobs that have arrived: KITE_ORDER_ARRIVAL (first come, first
+    // served) or KITE_ORDER_LEAST_WORK (least n^2 first: with a slot limit, short tests finish
+    // while long ones wait, which lowers the latency of most requests at the same throughput).
+    uint32_t order;
 } kite_run_opts;
 
 // Device bytes a job of this kind holds at length L while in flight (the admission estimate).
 uint64_t kite_job_bytes(uint32_t kind, uint32_t fft_len);
 
 // Runs the jobs through one batch: each job starts once its arrival time has passed and the slot
-// and HBM limits allow (first come, first served); every round advances all jobs in flight by up
+// and HBM limits allow (in the admission order of o->order); every round advances all jobs in flight by up
 // to `block` squarings with shared launches, then checks each (Fermat: Gerbicz every M blocks with
 // rollback, doubled tail; LLR: round-off after every block with a verified copy, confirmation
 // runs at the next FFT length and a vote as kite_llr). A finished job's slot is refilled at

@@ -207,6 +207,16 @@ def test_serving_arrivals_slots_and_horizon():
     assert out[1].status == kite.JOB_PENDING and rc == kite.JOB_PENDING
 
 
+def test_least_work_admission_order():
+    """KITE_ORDER_LEAST_WORK with one slot: jobs that arrive together start smallest n first, and
+    each result is the known one (tests/known_residues.csv)."""
+    jobs = [(kite.JOB_PRP, 5, 30000, 0.0), (kite.JOB_PRP, 8191, 25000, 0.0), (kite.JOB_PRP, 9357, 20000, 0.0)]
+    rc, out = kite.batch_run(jobs, slots=1, block=1000, order=kite.ORDER_LEAST_WORK)
+    assert rc == 0
+    assert out[2].start <= out[2].finish <= out[1].start <= out[1].finish <= out[0].start
+    assert ["%016X" % r.res64 for r in out] == ["B7980452FDBC4183", "D45538E6F07266C6", "79291CD2F47CE48C"]
+
+
 def test_hbm_budget_limits_jobs_in_flight():
     """With an HBM budget for two jobs, never more than two run at once; results unchanged."""
     L = kite.lib().kite_choose_fft_len(9357, 20000)
