@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -249,6 +250,18 @@ static ShapeEntry *get_shape(const ShapeChoice &ch) {
     return e;
 }
 
+// Testing hook (kite_debug_fail_hip): the n-th HIP call checked after it is set reports a
+// failure, so that error paths can be exercised without a device fault.
+static std::atomic<long> g_fail_countdown{0};
+
+// A HIP call's result, with the testing hook applied.
+static hipError_t hip_result(hipError_t e) {
+    if (e == hipSuccess && g_fail_countdown.load(std::memory_order_relaxed) > 0 &&
+        g_fail_countdown.fetch_sub(1) == 1)
+        return hipErrorUnknown;
+    return e;
+}
+
 struct Engine {
     hipStream_t stream = nullptr;
     ShapeEntry *shape = nullptr;
@@ -270,12 +283,28 @@ struct Engine {
     PassKernel kcfused = nullptr;   // square-mode fused column pass, if any
     unsigned long long *flags = nullptr;  // its per-chunk carry flags
     uint32_t seq = 0;                     // its launches so far
-    SlotDev *d_slot = nullptr;            // one-entry slot table read by the square graph
+    SlotDev *d_slot = nullptr;            // the slot table of this context's own launches (one entry)
     hipGraphExec_t sq_graph = nullptr;    // kSqGraphSteps squarings of the number in d_slot
     bool sq_graph_failed = false;
     std::string name[3];  // kernel names of the three passes, for timing reports
     uint32_t H = 0;
+    // first HIP failure of this context (sticky): every later operation is skipped, the round-off
+    // reads +inf and the drivers return a negative code
+    hipError_t err = hipSuccess;
+    std::string err_what;
 };
+
+// Records the first failure of E; true if `e` (and every earlier call) succeeded.
+static bool chk(Engine *E, hipError_t e, const char *what) {
+    e = hip_result(e);
+    if (e != hipSuccess && E->err == hipSuccess) {
+        E->err = e;
+        E->err_what = std::string(what) + ": " + hipGetErrorString(e);
+        g_err = E->err_what;
+        fprintf(stderr, "kite: %s\n", g_err.c_str());
+    }
+    return E->err == hipSuccess;
+}
 
 static std::once_flag g_attr_once;
 
@@ -342,7 +371,12 @@ int fill_slot_fold(kite_ctx *ctx, const ShapeDev &sh, SlotDev *s) {
     if (ctx->top_t == L) s->top_i0 = (int)(2 * sub);
     s->top_m = (int)ctx->top_m;
     s->top_shift = (int)(ctx->sL - ctx->top_st);
-    for (int i = 0; i < s->top_n; i++) s->top_bits[i] = ctx->bits[ctx->top_t + i];
+    s->top_big = 0;
+    for (int i = 0; i < s->top_n; i++) {
+        const int big = (int)ctx->bits[ctx->top_t + i] - (int)ctx->b0;
+        if (big < 0 || big > 1) return -1;
+        s->top_big |= (uint64_t)big << i;
+    }
     s->kscale = (double)ctx->k / (2.0 * L);  // k / (4H)
     s->rscale = 1.0 / sh.R;
     return 0;
@@ -386,7 +420,17 @@ int engine_init(kite_ctx *ctx) {
     HIP_OK(hipMalloc(&E->yfreq, (size_t)E->H * sizeof(double2)));
     HIP_OK(hipMalloc(&E->maxerr, 16));
     HIP_OK(hipMemset(E->maxerr, 0, 16));
+    HIP_OK(hipMalloc(&E->d_slot, sizeof(SlotDev)));
     SlotDev &s = E->slot;
+    s.R = sh.R;
+    s.C = sh.C;
+    s.G = sh.G;
+    s.NG = sh.NG;
+    s.twR = sh.twR;
+    s.twC = sh.twC;
+    s.twRows = sh.twRows;
+    s.twCols = sh.twCols;
+    s.tw16 = sh.tw16;
     s.shp = E->shape->dev;
     s.ta = E->ta;
     s.tb = E->tb;
@@ -412,8 +456,12 @@ int engine_init(kite_ctx *ctx) {
         E->krows_sq = (fr && ppw_for_C(sh.C) == 4) ? fast_rows_sq(sh.C) : nullptr;
         E->kcinv = fi ? fi : generic_cols_inv();
         E->kcfwd = ff ? ff : generic_cols_fwd();
-        E->kcfused = ff ? fast_cols_fused(sh.R) : nullptr;
-        E->lds_fused = ff ? fast_cols_fused_lds(sh.R) : 0;
+        // the fused pass passes carries between work-groups in 48-bit flag fields, which hold every
+        // carry when digits have at least kFusedMinBits bits; narrower digits take the split passes,
+        // whose carries are int64
+        const bool fused_ok = ff && ctx->b0 >= (uint32_t)kFusedMinBits;
+        E->kcfused = fused_ok ? fast_cols_fused(sh.R) : nullptr;
+        E->lds_fused = fused_ok ? fast_cols_fused_lds(sh.R) : 0;
         E->name[0] = E->krows_sq ? "k_rows_sq" + std::to_string(sh.C)
                      : fr       ? "k_rows_c<" + std::to_string(sh.C) + ">"
                                 : std::string("k_rows_g");
@@ -453,10 +501,17 @@ void engine_free(kite_ctx *ctx) {
     ctx->eng = nullptr;
 }
 
+int engine_error(const kite_ctx *ctx) { return ctx->eng && ctx->eng->err != hipSuccess ? -1 : 0; }
+
 void *engine_num_alloc(kite_ctx *ctx) {
+    Engine *E = ctx->eng;
     void *p = nullptr;
-    if (hipMalloc(&p, (size_t)ctx->eng->H * sizeof(double2)) != hipSuccess) return nullptr;
-    hipMemsetAsync(p, 0, (size_t)ctx->eng->H * sizeof(double2), ctx->eng->stream);
+    if (E->err != hipSuccess) return nullptr;
+    if (!chk(E, hipMalloc(&p, (size_t)E->H * sizeof(double2)), "hipMalloc(number)")) return nullptr;
+    if (!chk(E, hipMemsetAsync(p, 0, (size_t)E->H * sizeof(double2), E->stream), "hipMemsetAsync(number)")) {
+        hipFree(p);
+        return nullptr;
+    }
     return p;
 }
 
@@ -467,58 +522,17 @@ void engine_num_free(kite_ctx *ctx, void *p) {
 }
 
 void engine_copy(kite_ctx *ctx, void *dst, const void *src) {
-    hipMemcpyAsync(dst, src, (size_t)ctx->eng->H * sizeof(double2), hipMemcpyDeviceToDevice, ctx->eng->stream);
+    Engine *E = ctx->eng;
+    if (E->err != hipSuccess) return;
+    chk(E, hipMemcpyAsync(dst, src, (size_t)E->H * sizeof(double2), hipMemcpyDeviceToDevice, E->stream),
+        "hipMemcpyAsync(copy)");
 }
 
-static void launch_pass(PassKernel fn, int grid, size_t lds, hipStream_t st, const SlotDev *slots,
-                        const SlotDev &one, const int2 *jobs, int mode, int step, int threads = kNT) {
-    void *args[] = {(void *)&slots, (void *)&one, (void *)&jobs, (void *)&mode, (void *)&step};
-    hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st);
+static hipError_t launch_pass(PassKernel fn, int grid, size_t lds, hipStream_t st, const SlotDev *slots,
+                              const int2 *jobs, int mode, int step, int threads = kNT) {
+    void *args[] = {(void *)&slots, (void *)&jobs, (void *)&mode, (void *)&step};
+    return hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st);
 }
-
-// The passes read their slot from `tbl` (a device table, step `step` of a multi-step launch
-// sequence) or, with tbl null, from the by-value copy `s`.
-static void launch_rows(Engine *E, const SlotDev &s, int mode, const SlotDev *tbl = nullptr, int step = 0) {
-    if (mode == kModeSquare && E->krows_sq)
-        launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, tbl, s, nullptr, mode, step, kRowsSqThreads);
-    else
-        launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, tbl, s, nullptr, mode, step);
-}
-
-static void launch_rows(Engine *E, double2 *x, const double2 *ysrc, int mode) {
-    SlotDev s = E->slot;
-    s.x = x;
-    s.y_src = ysrc;
-    launch_rows(E, s, mode);
-}
-
-static void launch_cols(Engine *E, const SlotDev &s, int mode, const SlotDev *tbl = nullptr, int step = 0) {
-    launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, tbl, s, nullptr, mode, step);
-    launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, tbl, s, nullptr, mode, step);
-}
-
-static void launch_cols(Engine *E, double2 *x, int mode) {
-    SlotDev s = E->slot;
-    s.x = x;
-    launch_cols(E, s, mode);
-}
-
-// One squaring of this context's number alone: rows (one pair per work-group where that kernel
-// exists, since a single test leaves most CUs idle and latency decides), then the fused column
-// pass (fused step `step` uses seq s.seq + step + 1) or the two column passes.
-static void launch_square(Engine *E, const SlotDev &s, const SlotDev *tbl, int step) {
-    if (E->krows1) launch_pass(E->krows1, E->nrowjobs1, E->lds_rows1, E->stream, tbl, s, nullptr, kModeSquare, step);
-    else launch_rows(E, s, kModeSquare, tbl, step);
-    if (E->kcfused) launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, tbl, s, nullptr, kModeSquare, step);
-    else launch_cols(E, s, kModeSquare, tbl, step);
-}
-
-// Squarings of one context in runs of kSqGraphSteps are replayed from a captured HIP graph whose
-// kernels read the number, the launch counter and the step count from a one-entry device slot
-// table, written before each replay by a small kernel (its arguments are copied at launch, so no
-// host buffer has to outlive the call). Replays remove the host's per-launch cost, which bounds
-// a single test's squaring otherwise.
-constexpr uint32_t kSqGraphSteps = 32;
 
 __global__ void k_set_slot(SlotDev *dst, SlotDev v) {
     static_assert(sizeof(SlotDev) % 8 == 0 && sizeof(SlotDev) / 8 <= 64, "slot copied as 64-bit words");
@@ -526,24 +540,72 @@ __global__ void k_set_slot(SlotDev *dst, SlotDev v) {
     if (i < sizeof(SlotDev) / 8) reinterpret_cast<uint64_t *>(dst)[i] = reinterpret_cast<const uint64_t *>(&v)[i];
 }
 
+// This context's launches read their slot from E->d_slot, written here (in stream order) before
+// each operation: the kernels then load the slot's fields with scalar loads from one table.
+static bool set_slot(Engine *E, const SlotDev &s) {
+    if (E->err != hipSuccess) return false;
+    hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, E->stream, E->d_slot, s);
+    return chk(E, hipGetLastError(), "k_set_slot");
+}
+
+static bool launch_rows(Engine *E, int mode, int step = 0) {
+    if (mode == kModeSquare && E->krows_sq)
+        return chk(E, launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, E->d_slot, nullptr, mode, step,
+                                  kRowsSqThreads), "rows pass");
+    return chk(E, launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, E->d_slot, nullptr, mode, step),
+               "rows pass");
+}
+
+static bool launch_cols(Engine *E, int mode, int step = 0) {
+    return chk(E, launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, E->d_slot, nullptr, mode, step),
+               "column pass (inverse)") &&
+           chk(E, launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, E->d_slot, nullptr, mode, step),
+               "column pass (forward)");
+}
+
+// One squaring (step `step` of the slot in E->d_slot) of this context's number alone: rows (one
+// pair per work-group where that kernel exists, since a single test leaves most CUs idle and
+// latency decides), then the fused column pass (step s uses seq + s + 1) or the two column passes.
+static bool launch_square(Engine *E, int step) {
+    bool ok;
+    if (E->krows1)
+        ok = chk(E, launch_pass(E->krows1, E->nrowjobs1, E->lds_rows1, E->stream, E->d_slot, nullptr, kModeSquare,
+                                step), "rows pass");
+    else
+        ok = launch_rows(E, kModeSquare, step);
+    if (!ok) return false;
+    if (E->kcfused)
+        return chk(E, launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, E->d_slot, nullptr, kModeSquare,
+                                  step), "fused column pass");
+    return launch_cols(E, kModeSquare, step);
+}
+
+// Squarings of one context in runs of kSqGraphSteps are replayed from a captured HIP graph whose
+// kernels read the number, the launch counter and the step count from the slot in E->d_slot,
+// written before each replay. Replays remove the host's per-launch cost, which bounds a single
+// test's squaring otherwise.
+constexpr uint32_t kSqGraphSteps = 32;
+
 static bool build_square_graph(Engine *E) {
     if (E->sq_graph) return true;
-    if (E->sq_graph_failed) return false;
+    if (E->sq_graph_failed || E->err != hipSuccess) return false;
     E->sq_graph_failed = true;  // until the capture below succeeds
-    if (!E->d_slot && hipMalloc(&E->d_slot, sizeof(SlotDev)) != hipSuccess) {
-        E->d_slot = nullptr;
-        (void)hipGetLastError();
-        return false;
-    }
     hipGraph_t g = nullptr;
     bool ok = hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        for (uint32_t st = 0; st < kSqGraphSteps; st++) launch_square(E, E->slot, E->d_slot, (int)st);
-        ok = hipStreamEndCapture(E->stream, &g) == hipSuccess && g &&
+        bool launched = true;
+        for (uint32_t st = 0; st < kSqGraphSteps && launched; st++) {
+            const hipError_t saved = E->err;
+            launched = launch_square(E, (int)st);
+            E->err = saved;  // a launch refused during capture is not an engine failure
+        }
+        ok = hipStreamEndCapture(E->stream, &g) == hipSuccess && g && launched &&
              hipGraphInstantiate(&E->sq_graph, g, nullptr, nullptr, 0) == hipSuccess;
         if (g) hipGraphDestroy(g);
     }
-    (void)hipGetLastError();  // a failed capture falls back to direct launches
+    // a capture that failed (this clears only the capture's own error: the engine has none
+    // latched, checked above) falls back to direct launches
+    (void)hipGetLastError();
     if (!ok) E->sq_graph = nullptr;
     E->sq_graph_failed = !ok;
     return ok;
@@ -551,6 +613,7 @@ static bool build_square_graph(Engine *E) {
 
 void engine_square(kite_ctx *ctx, void *x, uint64_t count, int32_t xadd) {
     Engine *E = ctx->eng;
+    if (E->err != hipSuccess) return;
     SlotDev s = E->slot;
     s.x = (double2 *)x;
     s.xadd = xadd;
@@ -558,54 +621,60 @@ void engine_square(kite_ctx *ctx, void *x, uint64_t count, int32_t xadd) {
         s.count = kSqGraphSteps;
         for (; count >= kSqGraphSteps; count -= kSqGraphSteps) {
             s.seq = E->seq;
-            hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, E->stream, E->d_slot, s);
-            if (hipGraphLaunch(E->sq_graph, E->stream) != hipSuccess) {
-                // this run and the rest go through direct launches; the graph is not tried again
-                (void)hipGetLastError();
-                hipStreamSynchronize(E->stream);  // earlier replays finish before the graph goes
-                hipGraphExecDestroy(E->sq_graph);
-                E->sq_graph = nullptr;
-                E->sq_graph_failed = true;
-                break;
-            }
+            if (!set_slot(E, s)) return;
+            if (!chk(E, hipGraphLaunch(E->sq_graph, E->stream), "hipGraphLaunch(square)")) return;
             E->seq += kSqGraphSteps;
         }
-        s.count = E->slot.count;
     }
-    for (uint64_t i = 0; i < count; i++) {
-        s.seq = E->seq++;
-        launch_square(E, s, nullptr, 0);
+    // the rest by direct launches, in runs of up to 2^20 steps per slot write
+    while (count > 0) {
+        const uint32_t run = (uint32_t)std::min<uint64_t>(count, 1u << 20);
+        s.seq = E->seq;
+        s.count = run;
+        if (!set_slot(E, s)) return;
+        for (uint32_t i = 0; i < run; i++)
+            if (!launch_square(E, (int)i)) return;
+        E->seq += run;
+        count -= run;
     }
 }
 
 void engine_mul(kite_ctx *ctx, void *x, const void *y) {
     Engine *E = ctx->eng;
-    launch_rows(E, (double2 *)x, (const double2 *)y, kModeRowFft);
-    launch_rows(E, (double2 *)x, nullptr, kModeMul);
-    launch_cols(E, (double2 *)x, kModeMul);
+    SlotDev s = E->slot;
+    s.x = (double2 *)x;
+    s.y_src = (const double2 *)y;
+    if (!set_slot(E, s)) return;
+    if (launch_rows(E, kModeRowFft) && launch_rows(E, kModeMul)) launch_cols(E, kModeMul);
 }
 
 void engine_addsub(kite_ctx *ctx, void *x, const void *y, int sign) {
     Engine *E = ctx->eng;
+    if (E->err != hipSuccess) return;
     const int n = (int)E->H;
     hipLaunchKernelGGL(k_addsub, dim3((n + 255) / 256), dim3(256), 0, E->stream, (double2 *)x, (const double2 *)y, n,
                        (double)sign);
+    chk(E, hipGetLastError(), "k_addsub");
 }
 
 // digit j = 2 (r C + g G) + i  <->  blocked index (g R + r) 2G + i
 void engine_from_digits(kite_ctx *ctx, void *x, const int64_t *dig) {
     Engine *E = ctx->eng;
+    if (E->err != hipSuccess) return;
     const ShapeDev &sh = E->shape->host;
     const uint32_t R = sh.R, C = sh.C, G = sh.G, NG = sh.NG;
     for (uint32_t g = 0; g < NG; g++)
         for (uint32_t r = 0; r < R; r++)
             for (uint32_t i = 0; i < 2 * G; i++)
                 E->hdig[((size_t)g * R + r) * 2 * G + i] = (int32_t)dig[2 * ((size_t)r * C + g * G) + i];
-    hipMemcpyAsync(E->dig, E->hdig.data(), (size_t)ctx->L * 4, hipMemcpyHostToDevice, E->stream);
     SlotDev s = E->slot;
     s.x = (double2 *)x;
-    launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, kModeDigits, 0);
-    hipStreamSynchronize(E->stream);  // hdig is reused
+    if (chk(E, hipMemcpyAsync(E->dig, E->hdig.data(), (size_t)ctx->L * 4, hipMemcpyHostToDevice, E->stream),
+            "hipMemcpyAsync(digits to device)") &&
+        set_slot(E, s))
+        chk(E, launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, E->d_slot, nullptr, kModeDigits, 0),
+            "column pass (from digits)");
+    chk(E, hipStreamSynchronize(E->stream), "hipStreamSynchronize(from digits)");  // hdig is reused
 }
 
 void engine_to_digits(kite_ctx *ctx, const void *x, int64_t *dig) {
@@ -615,9 +684,16 @@ void engine_to_digits(kite_ctx *ctx, const void *x, int64_t *dig) {
     s.x = (double2 *)const_cast<void *>(x);
     // the conversion's own rounding error is not an operation's round-off: count it aside
     s.maxerr = E->maxerr + 1;
-    launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, nullptr, s, nullptr, kModeDigits, 0);
-    hipMemcpyAsync(E->hdig.data(), E->dig, (size_t)ctx->L * 4, hipMemcpyDeviceToHost, E->stream);
-    hipStreamSynchronize(E->stream);
+    bool ok = set_slot(E, s) &&
+              chk(E, launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, E->d_slot, nullptr, kModeDigits, 0),
+                  "column pass (to digits)") &&
+              chk(E, hipMemcpyAsync(E->hdig.data(), E->dig, (size_t)ctx->L * 4, hipMemcpyDeviceToHost, E->stream),
+                  "hipMemcpyAsync(digits to host)");
+    ok = chk(E, hipStreamSynchronize(E->stream), "hipStreamSynchronize(to digits)") && ok;
+    if (!ok) {  // nothing valid came back: zeros, with the failure latched for the caller
+        memset(dig, 0, (size_t)ctx->L * sizeof(int64_t));
+        return;
+    }
     const uint32_t R = sh.R, C = sh.C, G = sh.G, NG = sh.NG;
     for (uint32_t g = 0; g < NG; g++)
         for (uint32_t r = 0; r < R; r++)
@@ -625,20 +701,29 @@ void engine_to_digits(kite_ctx *ctx, const void *x, int64_t *dig) {
                 dig[2 * ((size_t)r * C + g * G) + i] = E->hdig[((size_t)g * R + r) * 2 * G + i];
 }
 
+// Max round-off since the last reset; +inf once the context has failed, so that every caller's
+// round-off test fails as well.
 double engine_maxerr(kite_ctx *ctx) {
+    Engine *E = ctx->eng;
     unsigned long long v = 0;
-    hipMemcpyAsync(&v, ctx->eng->maxerr, 8, hipMemcpyDeviceToHost, ctx->eng->stream);
-    hipStreamSynchronize(ctx->eng->stream);
+    if (E->err == hipSuccess) {
+        chk(E, hipMemcpyAsync(&v, E->maxerr, 8, hipMemcpyDeviceToHost, E->stream), "hipMemcpyAsync(round-off)");
+        chk(E, hipStreamSynchronize(E->stream), "hipStreamSynchronize(round-off)");
+    }
+    if (E->err != hipSuccess) return INFINITY;
     double d;
     memcpy(&d, &v, 8);
     return d;
 }
 
 void engine_reset_maxerr(kite_ctx *ctx) {
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->eng->stream, ctx->eng->maxerr, 0ull, 1);
+    Engine *E = ctx->eng;
+    if (E->err != hipSuccess) return;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, E->stream, E->maxerr, 0ull, 1);
+    chk(E, hipGetLastError(), "k_set_u64");
 }
 
-void engine_sync(kite_ctx *ctx) { hipStreamSynchronize(ctx->eng->stream); }
+void engine_sync(kite_ctx *ctx) { chk(ctx->eng, hipStreamSynchronize(ctx->eng->stream), "hipStreamSynchronize"); }
 
 }  // namespace kite
 
@@ -704,31 +789,70 @@ struct kite_batch {
         uint64_t launches = 0;
     };
     std::vector<KStat> kstats;
+    // first HIP failure of the batch (sticky): every later call returns -1
+    hipError_t err = hipSuccess;
 };
 
 using namespace kite;
 
-static void batch_sync_all(kite_batch *b) {
+// Records the first failure of b; true if `e` (and every earlier call) succeeded.
+static bool bchk(kite_batch *b, hipError_t e, const char *what) {
+    e = hip_result(e);
+    if (e != hipSuccess && b->err == hipSuccess) {
+        b->err = e;
+        g_err = std::string(what) + ": " + hipGetErrorString(e);
+        fprintf(stderr, "kite: batch: %s\n", g_err.c_str());
+    }
+    return b->err == hipSuccess;
+}
+
+#define BATCH_OK(b, expr)                                 \
+    do {                                                  \
+        if (!bchk((b), (expr), #expr)) return -1;         \
+    } while (0)
+
+static bool batch_sync_all(kite_batch *b) {
+    bool ok = true;
     for (int p = 0; p < kite_batch::kMaxParts; p++)
-        if (b->ps[p]) hipStreamSynchronize(b->ps[p]);
+        if (b->ps[p]) ok = bchk(b, hipStreamSynchronize(b->ps[p]), "hipStreamSynchronize(batch)") && ok;
+    return ok;
+}
+
+// The row kernel and job count a slot's row pass uses: a batch of one test spreads its rows over
+// one pair per work-group where that kernel exists (latency decides when the device is mostly idle).
+static void rows_kernel_of(const Engine *E, bool alone, PassKernel *k, PassKernel *ksq, int *njobs, size_t *lds) {
+    if (alone && E->krows1) {
+        *k = E->krows1;
+        *ksq = nullptr;
+        *njobs = E->nrowjobs1;
+        *lds = E->lds_rows1;
+    } else {
+        *k = E->krows;
+        *ksq = E->krows_sq;
+        *njobs = E->nrowjobs;
+        *lds = E->lds_rows;
+    }
 }
 
 static int batch_refresh(kite_batch *b) {
+    if (b->err != hipSuccess) return -1;
     if (!b->dirty) return 0;
-    batch_sync_all(b);  // the tables below may be reallocated
+    if (!batch_sync_all(b)) return -1;  // the tables below may be reallocated
     const size_t cap = b->slots.size();
     if (cap > b->slot_cap) {
         const size_t nc = std::max<size_t>(cap, 2 * b->slot_cap);
         if (b->d_slots) hipFree(b->d_slots);
         if (b->h_slots) hipHostFree(b->h_slots);
+        b->d_slots = b->h_slots = nullptr;
         unsigned long long *old = b->d_maxerr;
-        HIP_OK(hipMalloc(&b->d_slots, nc * sizeof(SlotDev)));
-        HIP_OK(hipHostMalloc((void **)&b->h_slots, nc * sizeof(SlotDev), hipHostMallocDefault));
-        HIP_OK(hipMalloc(&b->d_maxerr, nc * 8));
-        HIP_OK(hipMemsetAsync(b->d_maxerr, 0, nc * 8, b->stream));
+        b->d_maxerr = nullptr;
+        BATCH_OK(b, hipMalloc(&b->d_slots, nc * sizeof(SlotDev)));
+        BATCH_OK(b, hipHostMalloc((void **)&b->h_slots, nc * sizeof(SlotDev), hipHostMallocDefault));
+        BATCH_OK(b, hipMalloc(&b->d_maxerr, nc * 8));
+        BATCH_OK(b, hipMemsetAsync(b->d_maxerr, 0, nc * 8, b->stream));
         if (old) {
-            HIP_OK(hipMemcpyAsync(b->d_maxerr, old, b->slot_cap * 8, hipMemcpyDeviceToDevice, b->stream));
-            HIP_OK(hipStreamSynchronize(b->stream));
+            BATCH_OK(b, hipMemcpyAsync(b->d_maxerr, old, b->slot_cap * 8, hipMemcpyDeviceToDevice, b->stream));
+            BATCH_OK(b, hipStreamSynchronize(b->stream));
             hipFree(old);
         }
         b->slot_cap = nc;
@@ -740,6 +864,7 @@ static int batch_refresh(kite_batch *b) {
     std::vector<int> order;
     for (size_t s = 0; s < cap; s++)
         if (b->slots[s].ctx) order.push_back((int)s);
+    const bool alone = order.size() == 1;
     auto stat_of = [&](const std::string &nm, int pass) {
         for (size_t q = 0; q < b->kstats.size(); q++)
             if (b->kstats[q].name == nm && b->kstats[q].pass == pass) return (int)q;
@@ -770,21 +895,30 @@ static int batch_refresh(kite_batch *b) {
         for (int s0 : ord) {
             if (done[s0]) continue;
             Engine *E0 = b->slots[s0].ctx->eng;
-            const PassKernel key = isrows ? E0->krows : E0->kcinv;
-            kite_batch::Group g{key, isrows ? E0->krows_sq : E0->kcfwd, (int)jobs.size(), 0, 0, 0.0, {-1, -1}};
+            PassKernel rk0, rsq0;
+            int rj0;
+            size_t rl0;
+            rows_kernel_of(E0, alone, &rk0, &rsq0, &rj0, &rl0);
+            const PassKernel key = isrows ? rk0 : E0->kcinv;
+            kite_batch::Group g{key, isrows ? rsq0 : E0->kcfwd, (int)jobs.size(), 0, 0, 0.0, {-1, -1}};
             for (int s : ord) {
                 Engine *E = b->slots[s].ctx->eng;
-                if (done[s] || (isrows ? E->krows : E->kcinv) != key) continue;
+                PassKernel rk, rsq;
+                int rj;
+                size_t rl;
+                rows_kernel_of(E, alone, &rk, &rsq, &rj, &rl);
+                if (done[s] || (isrows ? rk : E->kcinv) != key || (!isrows && E->kcfused != E0->kcfused)) continue;
                 done[s] = true;
-                const int nj = isrows ? E->nrowjobs : E->ncoljobs;
+                const int nj = isrows ? rj : E->ncoljobs;
                 for (int u = 0; u < nj; u++) jobs.push_back(make_int2(s, u));
-                g.lds = std::max(g.lds, isrows ? E->lds_rows : E->lds_cols);
+                g.lds = std::max(g.lds, isrows ? rl : E->lds_cols);
                 if (!isrows) g.lds_f = std::max(g.lds_f, E->lds_fused);
                 g.sumL += 2.0 * E->H;
             }
             g.count = (int)jobs.size() - g.off;
             if (isrows) {
-                g.stat[0] = stat_of(E0->name[0], 0);
+                g.stat[0] = stat_of(rk0 == E0->krows1 && alone ? "k_rows_c1<" + std::to_string(E0->shape->ch.C) + ">"
+                                                               : E0->name[0], 0);
             } else {
                 g.stat[0] = stat_of(E0->name[1], 1);
                 g.stat[1] = stat_of(E0->name[2], 2);
@@ -800,12 +934,13 @@ static int batch_refresh(kite_batch *b) {
             out.insert(out.end(), grp.begin(), grp.end());
             return;
         }
-        // adjacent compile-time column lengths share one launch per pass
+        // adjacent compile-time column lengths share one launch per pass (the fused pass only if
+        // both lengths have it)
         std::vector<kite_batch::Group> merged;
         for (size_t i = 0; i < grp.size(); i++) {
             const int r1 = gR[i];
             if (i + 1 < grp.size() && grp[i].k1 == fast_cols_inv(r1) && fast_cols_next(r1) == gR[i + 1] &&
-                grp[i + 1].k1 == fast_cols_inv(gR[i + 1])) {
+                grp[i + 1].k1 == fast_cols_inv(gR[i + 1]) && (grp[i].kf != nullptr) == (grp[i + 1].kf != nullptr)) {
                 kite_batch::Group g = grp[i];
                 g.k1 = fast_cols_pair_inv(r1);
                 g.k2 = fast_cols_pair_fwd(r1);
@@ -834,13 +969,14 @@ static int batch_refresh(kite_batch *b) {
     if (need > b->jobs_cap) {
         if (b->d_rows) hipFree(b->d_rows);
         if (b->d_cols) hipFree(b->d_cols);
+        b->d_rows = b->d_cols = nullptr;
         const size_t nc = std::max(need, 2 * b->jobs_cap);
-        HIP_OK(hipMalloc(&b->d_rows, nc * sizeof(int2)));
-        HIP_OK(hipMalloc(&b->d_cols, nc * sizeof(int2)));
+        BATCH_OK(b, hipMalloc(&b->d_rows, nc * sizeof(int2)));
+        BATCH_OK(b, hipMalloc(&b->d_cols, nc * sizeof(int2)));
         b->jobs_cap = nc;
     }
-    if (!rows.empty()) HIP_OK(hipMemcpy(b->d_rows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
-    if (!cols.empty()) HIP_OK(hipMemcpy(b->d_cols, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!rows.empty()) BATCH_OK(b, hipMemcpy(b->d_rows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!cols.empty()) BATCH_OK(b, hipMemcpy(b->d_cols, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
     b->nrows = (int)rows.size();
     b->ncols = (int)cols.size();
     b->dirty = false;
@@ -851,7 +987,7 @@ static int batch_refresh(kite_batch *b) {
 // Fills the slot table: residue `x` (and operand `y`) per slot, step count per slot (0 = idle).
 static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, const uint32_t *count) {
     if (batch_refresh(b)) return -1;
-    batch_sync_all(b);  // the previous table may still be read
+    if (!batch_sync_all(b)) return -1;  // the previous table may still be read
     for (size_t s = 0; s < b->slots.size(); s++) {
         SlotDev &d = b->h_slots[s];
         kite_ctx *ctx = b->slots[s].ctx;
@@ -868,20 +1004,19 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
         d.x = on ? (double2 *)b->slots[s].nums[xi[s]]->dev : nullptr;
         d.y_src = (on && yi && yi[s] >= 0) ? (const double2 *)b->slots[s].nums[yi[s]]->dev : nullptr;
     }
-    HIP_OK(hipMemcpyAsync(b->d_slots, b->h_slots, b->slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice,
-                          b->stream));
+    BATCH_OK(b, hipMemcpyAsync(b->d_slots, b->h_slots, b->slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice,
+                               b->stream));
     // the other parts' launches read the table too: they wait for the copy
     if (b->nparts > 1) {
-        HIP_OK(hipEventRecord(b->evtable, b->stream));
-        for (int p = 1; p < b->nparts; p++) HIP_OK(hipStreamWaitEvent(b->ps[p], b->evtable, 0));
+        BATCH_OK(b, hipEventRecord(b->evtable, b->stream));
+        for (int p = 1; p < b->nparts; p++) BATCH_OK(b, hipStreamWaitEvent(b->ps[p], b->evtable, 0));
     }
     return 0;
 }
 
 // One launch per group of the pass; with `timed`, each launch is bracketed by an event pair.
 // In square mode a column group with a fused kernel runs it as pass 1 and skips pass 2.
-static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed = false) {
-    SlotDev none{};
+static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool timed = false) {
     const auto &groups = pass == 0 ? b->grows : b->gcols;
     const int2 *jobs = pass == 0 ? b->d_rows : b->d_cols;
     for (const auto &g : groups) {
@@ -892,21 +1027,24 @@ static void batch_launch(kite_batch *b, int pass, int mode, int step, bool timed
         const PassKernel fn = fused ? g.kf : (pass == 2 || rsq) ? g.k2 : g.k1;
         const size_t lds = rsq ? 0 : fused ? g.lds_f : g.lds;
         const int threads = rsq ? kRowsSqThreads : kNT;
+        hipError_t e;
         if (timed) {
-            const size_t e = 2 * b->ev_stat.size();
-            if (b->ev.size() < e + 2) {
-                b->ev.resize(e + 2);
-                hipEventCreate(&b->ev[e]);
-                hipEventCreate(&b->ev[e + 1]);
+            const size_t ei = 2 * b->ev_stat.size();
+            if (b->ev.size() < ei + 2) {
+                b->ev.resize(ei + 2);
+                hipEventCreate(&b->ev[ei]);
+                hipEventCreate(&b->ev[ei + 1]);
             }
             b->ev_stat.push_back(fused ? g.stat_f : g.stat[pass == 2 ? 1 : 0]);
-            hipEventRecord(b->ev[e], b->ps[g.part]);
-            launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, none, jobs + g.off, mode, step, threads);
-            hipEventRecord(b->ev[e + 1], b->ps[g.part]);
+            hipEventRecord(b->ev[ei], b->ps[g.part]);
+            e = launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, jobs + g.off, mode, step, threads);
+            hipEventRecord(b->ev[ei + 1], b->ps[g.part]);
         } else {
-            launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, none, jobs + g.off, mode, step, threads);
+            e = launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, jobs + g.off, mode, step, threads);
         }
+        if (e != hipSuccess) return e;
     }
+    return hipSuccess;
 }
 
 extern "C" kite_batch *kite_batch_new(void) {
@@ -948,6 +1086,7 @@ extern "C" void kite_batch_free(kite_batch *b) {
 }
 
 extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t fft_len, uint32_t nnums) {
+    if (b->err != hipSuccess) return -1;
     kite_opts o{};
     o.backend = KITE_BACKEND_HIP;
     o.fft_len = fft_len;
@@ -964,7 +1103,12 @@ extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t ff
         }
         bs.nums.push_back(x);
     }
-    hipStreamSynchronize(ctx->eng->stream);
+    engine_sync(ctx);
+    if (engine_error(ctx)) {
+        for (kite_num *y : bs.nums) kite_num_free(ctx, y);
+        kite_ctx_free(ctx);
+        return -1;
+    }
     size_t s = 0;
     while (s < b->slots.size() && b->slots[s].ctx) s++;
     if (s == b->slots.size()) b->slots.emplace_back();
@@ -973,7 +1117,8 @@ extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t ff
     if (batch_refresh(b)) return -1;
     // clear this slot's round-off accumulator
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, b->stream, b->d_maxerr + s, 0ull, 1);
-    hipStreamSynchronize(b->stream);
+    BATCH_OK(b, hipGetLastError());
+    BATCH_OK(b, hipStreamSynchronize(b->stream));
     return (int)s;
 }
 
@@ -998,9 +1143,12 @@ extern "C" kite_num *kite_batch_num(kite_batch *b, int slot, uint32_t i) {
     return b->slots[slot].nums[i];
 }
 
+// 0 if the batch and every slot's context (whose streams carry the limb conversions) are intact.
 extern "C" int kite_batch_sync(kite_batch *b) {
-    batch_sync_all(b);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (!batch_sync_all(b)) return -1;
+    for (const BatchSlot &bs : b->slots)
+        if (bs.ctx && engine_error(bs.ctx)) return -1;
+    return 0;
 }
 
 extern "C" int kite_batch_set_parts(kite_batch *b, int parts) {
@@ -1015,6 +1163,7 @@ extern "C" int kite_batch_set_parts(kite_batch *b, int parts) {
 }
 
 extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint32_t *count) {
+    if (b->err != hipSuccess) return -1;
     uint32_t maxc = 0;
     for (size_t s = 0; s < b->slots.size(); s++)
         if (b->slots[s].ctx && which[s] >= 0) maxc = std::max(maxc, count[s]);
@@ -1023,13 +1172,15 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
     b->ev_stat.clear();
     for (size_t s = 0; s < b->slots.size(); s++)
         if (b->slots[s].ctx) b->slots[s].ctx->eng->seq += b->h_slots[s].count;  // the table holds the old value
-    auto launch_all = [&]() {
+    auto launch_all = [&]() -> hipError_t {
         for (uint32_t st = 0; st < maxc; st++) {
             const bool timed = b->timing && st % (uint32_t)b->timing == 0;
-            batch_launch(b, 0, kModeSquare, (int)st, timed);
-            batch_launch(b, 1, kModeSquare, (int)st, timed);
-            batch_launch(b, 2, kModeSquare, (int)st, timed);
+            for (int pass = 0; pass < 3; pass++) {
+                const hipError_t e = batch_launch(b, pass, kModeSquare, (int)st, timed);
+                if (e != hipSuccess) return e;
+            }
         }
+        return hipSuccess;
     };
     // Kernels read the slot table at run time, so a captured sequence of maxc steps stays valid
     // until the job arrays change (a new generation): the usual Gerbicz-block loop replays it.
@@ -1043,17 +1194,19 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
             // the other parts' streams join the capture (fork) and rejoin the first (join)
             hipEventRecord(b->evfork, b->stream);
             for (int p = 1; p < b->nparts; p++) hipStreamWaitEvent(b->ps[p], b->evfork, 0);
-            launch_all();
+            const bool launched = launch_all() == hipSuccess;
             for (int p = 1; p < b->nparts; p++) {
                 hipEventRecord(b->evjoin[p], b->ps[p]);
                 hipStreamWaitEvent(b->stream, b->evjoin[p], 0);
             }
-            replay = hipStreamEndCapture(b->stream, &g) == hipSuccess && g &&
+            replay = hipStreamEndCapture(b->stream, &g) == hipSuccess && g && launched &&
                      hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) == hipSuccess;
             if (g) hipGraphDestroy(g);
             if (!replay) b->gexec = nullptr;
         }
-        (void)hipGetLastError();  // a failed capture falls back to direct launches
+        // a capture that failed falls back to direct launches (this clears only the capture's own
+        // error: every earlier call of the batch was checked)
+        (void)hipGetLastError();
         b->gsteps = maxc;
         b->ggen = b->gen;
     }
@@ -1061,19 +1214,19 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
         // the graph runs every part on the first stream: order it after the other streams' work,
         // and their later work (copies) after it
         for (int p = 1; p < b->nparts; p++) {
-            hipEventRecord(b->evjoin[p], b->ps[p]);
-            hipStreamWaitEvent(b->stream, b->evjoin[p], 0);
+            BATCH_OK(b, hipEventRecord(b->evjoin[p], b->ps[p]));
+            BATCH_OK(b, hipStreamWaitEvent(b->stream, b->evjoin[p], 0));
         }
-        if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return -1;
-        hipEventRecord(b->evfork, b->stream);
-        for (int p = 1; p < b->nparts; p++) hipStreamWaitEvent(b->ps[p], b->evfork, 0);
+        BATCH_OK(b, hipGraphLaunch(b->gexec, b->stream));
+        BATCH_OK(b, hipEventRecord(b->evfork, b->stream));
+        for (int p = 1; p < b->nparts; p++) BATCH_OK(b, hipStreamWaitEvent(b->ps[p], b->evfork, 0));
     } else {
-        launch_all();
+        BATCH_OK(b, launch_all());
     }
     if (b->timing) {
         // per-pass and per-kernel totals of this call; algorithmic bytes per test and launch are
         // 16 L for the row pass (residue read + written) and 8 L for each column pass
-        batch_sync_all(b);
+        if (!batch_sync_all(b)) return -1;
         for (int p = 0; p < 3; p++) {
             b->ms[p] = 0;
             b->launches[p] = (maxc + b->timing - 1) / b->timing;
@@ -1098,20 +1251,21 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
             b->ms[ks.pass] += ms;
         }
     }
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    return 0;
 }
 
 extern "C" int kite_batch_mul(kite_batch *b, const int32_t *dst, const int32_t *src) {
+    if (b->err != hipSuccess) return -1;
     std::vector<uint32_t> one(b->slots.size(), 1);
     bool any = false;
     for (size_t s = 0; s < b->slots.size(); s++) any |= b->slots[s].ctx && dst[s] >= 0;
     if (!any) return 0;
     if (batch_upload(b, dst, src, one.data())) return -1;
-    batch_launch(b, 0, kModeRowFft, 0);
-    batch_launch(b, 0, kModeMul, 0);
-    batch_launch(b, 1, kModeMul, 0);
-    batch_launch(b, 2, kModeMul, 0);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    BATCH_OK(b, batch_launch(b, 0, kModeRowFft, 0));
+    BATCH_OK(b, batch_launch(b, 0, kModeMul, 0));
+    BATCH_OK(b, batch_launch(b, 1, kModeMul, 0));
+    BATCH_OK(b, batch_launch(b, 2, kModeMul, 0));
+    return 0;
 }
 
 extern "C" int kite_batch_copy(kite_batch *b, const int32_t *dst, const int32_t *src) {
@@ -1119,10 +1273,11 @@ extern "C" int kite_batch_copy(kite_batch *b, const int32_t *dst, const int32_t 
     for (size_t s = 0; s < b->slots.size(); s++) {
         BatchSlot &bs = b->slots[s];
         if (!bs.ctx || dst[s] < 0 || dst[s] == src[s]) continue;
-        hipMemcpyAsync(bs.nums[dst[s]]->dev, bs.nums[src[s]]->dev, (size_t)bs.ctx->eng->H * sizeof(double2),
-                       hipMemcpyDeviceToDevice, b->ps[b->slot_part[s]]);  // ordered with the slot's launches
+        // ordered with the slot's launches
+        BATCH_OK(b, hipMemcpyAsync(bs.nums[dst[s]]->dev, bs.nums[src[s]]->dev, (size_t)bs.ctx->eng->H * sizeof(double2),
+                                   hipMemcpyDeviceToDevice, b->ps[b->slot_part[s]]));
     }
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    return 0;
 }
 
 extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
@@ -1130,14 +1285,15 @@ extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
     const size_t cap = b->slots.size();
     if (!cap) return 0;
     std::vector<unsigned long long> v(cap);
-    batch_sync_all(b);
-    HIP_OK(hipMemcpyAsync(v.data(), b->d_maxerr, cap * 8, hipMemcpyDeviceToHost, b->stream));
-    HIP_OK(hipStreamSynchronize(b->stream));
+    if (!batch_sync_all(b)) return -1;
+    BATCH_OK(b, hipMemcpyAsync(v.data(), b->d_maxerr, cap * 8, hipMemcpyDeviceToHost, b->stream));
+    BATCH_OK(b, hipStreamSynchronize(b->stream));
     for (size_t s = 0; s < cap; s++) memcpy(&out[s], &v[s], 8);
     if (reset) {
         hipLaunchKernelGGL(k_set_u64, dim3((unsigned)((cap + 63) / 64)), dim3(64), 0, b->stream, b->d_maxerr, 0ull,
                            (int)cap);
-        HIP_OK(hipStreamSynchronize(b->stream));
+        BATCH_OK(b, hipGetLastError());
+        BATCH_OK(b, hipStreamSynchronize(b->stream));
     }
     return 0;
 }
@@ -1180,3 +1336,7 @@ extern "C" int kite_batch_kernel_times(kite_batch *b, int i, char *name, size_t 
     if (bytes) *bytes = ks.bytes;
     return (int)b->kstats.size();
 }
+
+// Testing hook: the n-th HIP call checked from now on (by any context or batch) reports a failure
+// (n <= 0 turns it off). No device operation fails; the library only takes the failure path.
+extern "C" void kite_debug_fail_hip(long n) { g_fail_countdown.store(n > 0 ? n : 0); }

@@ -17,6 +17,8 @@ constexpr int kMaxTop = 64;         // top-region digits handled by the device f
 constexpr int kSub = 8;             // elements (16 digits) per carry sub-chunk
 constexpr int kTwLdsMaxC = 1024;    // row twiddles are staged in LDS up to this row length
 constexpr int kRowPad = 3;          // row-kernel LDS: one padding element per 8
+constexpr int kFusedMinBits = 6;    // fused column pass: digits of at least this many bits (carry < 2^47)
+constexpr int kRowsSqThreads = 128; // threads per work-group of the register row kernel (C = 256)
 
 // Radix plan of a length-n FFT: 8s first, then 4, 2, 3, 5, 7 (host make_plan uses the same order).
 __host__ __device__ constexpr int ct_nstages(int n) {
@@ -74,7 +76,11 @@ struct ShapeDev {  // one (R, C, G) shape: H = R*C complex, NG = C/G column grou
 };
 
 struct SlotDev {  // one number being operated on (one test in a batch)
-    const ShapeDev *shp;
+    // the shape's sizes and tables, copied here so that a kernel reads them with the slot (one
+    // dependent load fewer than through shp)
+    int R, C, G, NG;
+    const double2 *twR, *twC, *twRows, *twCols, *tw16;
+    const ShapeDev *shp;        // the whole shape (plans, two-level w_H table) for the generic kernels
     double2 *x;                 // the residue, H complex (after forward column pass)
     const double2 *y_src;       // multiplication operand (stored form), for kModeRowFft
     double2 *yfreq;             // operand row spectra, H complex
@@ -84,7 +90,7 @@ struct SlotDev {  // one number being operated on (one test in a batch)
     const double *ta;           // [R] 2^-fa(r), [R] 2^+fa(r)
     const double *tb;           // [2C] 2^-fb(b), [2C] 2^+fb(b)
     unsigned long long *maxerr; // max round-off (bits of a non-negative double)
-    unsigned long long *flags;  // fused column pass: per (g, r) chunk, seq << 40 | carry (40 bits)
+    unsigned long long *flags;  // fused column pass: per (g, r) chunk, seq << 48 | carry (48 bits)
     uint32_t seq;               // fused column launches of this slot so far (step s uses seq + s + 1)
     int32_t xadd;               // square mode: x -> x^2 + xadd (V += xadd 2^n, added in the top fold)
     double kscale;              // k / (4H)
@@ -95,7 +101,7 @@ struct SlotDev {  // one number being operated on (one test in a batch)
     int b0;                     // digits have b0 or b0+1 bits
     int top_i0, top_n, top_m, top_shift;  // fold: first top digit within the last chunk, count,
                                           // m = n - s_t, s_L - s_t
-    uint8_t top_bits[kMaxTop];
+    uint64_t top_big;           // bit i: top digit i has b0 + 1 bits (else b0)
 };
 
 }  // namespace kite

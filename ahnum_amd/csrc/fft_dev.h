@@ -320,6 +320,37 @@ __device__ __forceinline__ double2 *fft_ct(double2 *a, double2 *b, const double2
     }
 }
 
+// Loads and stores of device memory through global-address-space pointers: global (not flat)
+// instructions, which wait on the memory counter only.
+typedef double dv2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 gld(const double2 *p) {
+    const dv2_t v = *(const __attribute__((address_space(1))) dv2_t *)p;
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(double2 *p, double2 v) {
+    dv2_t w;
+    w.x = v.x;
+    w.y = v.y;
+    *(__attribute__((address_space(1))) dv2_t *)p = w;
+}
+// Scalars and int4 through global-address-space pointers (the tables a kernel copies into LDS: a
+// flat load could alias LDS, so the compiler would wait for each one before the next LDS write).
+template <class T> __device__ __forceinline__ T ldg(const T *p) { return *(const __attribute__((address_space(1))) T *)p; }
+template <class T> __device__ __forceinline__ void stg(T *p, T v) { *(__attribute__((address_space(1))) T *)p = v; }
+typedef int iv4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 gld4(const int32_t *p) {
+    const iv4_t v = *(const __attribute__((address_space(1))) iv4_t *)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst4(int32_t *p, int4 v) {
+    iv4_t w;
+    w.x = v.x;
+    w.y = v.y;
+    w.z = v.z;
+    w.w = v.w;
+    *(__attribute__((address_space(1))) iv4_t *)p = w;
+}
+
 // w_H^t for t < H from the two-level table
 __device__ __forceinline__ double2 tw_h(const ShapeDev &sh, int t) {
     return cmul(sh.thi[t >> sh.sshift], sh.tlo[t & ((1 << sh.sshift) - 1)]);
@@ -349,10 +380,6 @@ __device__ __forceinline__ void split_mul(double2 &a, double2 &b, double2 t, dou
     double2 Y = cadd(cmul(e, oy), cmul(o, ey));
     a = make_double2(X.x - Y.y, X.y + Y.x);
     b = make_double2(X.x + Y.y, Y.x - X.y);
-}
-
-__device__ __forceinline__ const SlotDev &pick_slot(const SlotDev *slots, const SlotDev &one, int i) {
-    return slots ? slots[i] : one;
 }
 
 // floor(t / k) for t < 2^64, 0 < k < 2^32, inv = floor((2^64 - 1) / k): inv >= (2^64 - k) / k, so
@@ -390,7 +417,7 @@ __device__ __forceinline__ int64_t fold_top_dev(const SlotDev &s, int32_t *d, in
     int pos = 0;
     for (int i = 0; i < nt; i++) {
         Hh += (i128)d[t0 + i] << pos;
-        pos += s.top_bits[i];
+        pos += s.b0 + (int)((s.top_big >> i) & 1);
     }
     const i128 K = (i128)s.k;
     const i128 lo = Hh & (((i128)1 << m) - 1), hi = Hh >> m;
@@ -401,7 +428,7 @@ __device__ __forceinline__ int64_t fold_top_dev(const SlotDev &s, int32_t *d, in
         q += 1;
     }
     for (int i = 0; i + 1 < nt; i++) {
-        const int b = s.top_bits[i];
+        const int b = s.b0 + (int)((s.top_big >> i) & 1);
         const i128 qq = (top + ((i128)1 << (b - 1))) >> b;
         d[t0 + i] = (int32_t)(top - (qq << b));
         top = qq;
@@ -433,11 +460,11 @@ __device__ __forceinline__ int64_t bal(int64_t v, int b, int32_t &d) {
 // for odd R), pair p > 0 is (p, R-p). LDS holds 2*PPW rows of C (row slot 2i+h) twice.
 
 template <int CC>
-__device__ __forceinline__ void rows_body(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+__device__ __forceinline__ void rows_body(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                           int step) {
     extern __shared__ double2 lds[];
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
+    const SlotDev &s = slots[jb.x];
     if (step >= (int)s.count) return;  // this slot sits out this launch (uniform per work-group)
     const ShapeDev &sh = *s.shp;
     constexpr int PPW_CT = CC ? ppw_for_C(CC) : 1;
@@ -521,11 +548,11 @@ __device__ __forceinline__ void rows_body(const SlotDev *slots, const SlotDev &o
 // sub-chunk. Digits are staged as int32 [r][2G] in whichever of A/B is free.
 
 template <int RC>
-__device__ __forceinline__ void cols_inv_body(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+__device__ __forceinline__ void cols_inv_body(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                               int step) {
     extern __shared__ double2 lds[];
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
+    const SlotDev &s = slots[jb.x];
     if (step >= (int)s.count) return;
     const ShapeDev &sh = *s.shp;
     constexpr int G_CT = RC ? g_for_R(RC) : 0;
@@ -623,11 +650,11 @@ __device__ __forceinline__ void cols_inv_body(const SlotDev *slots, const SlotDe
 }
 
 template <int RC>
-__device__ __forceinline__ void cols_fwd_body(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+__device__ __forceinline__ void cols_fwd_body(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                               int step) {
     extern __shared__ double2 lds[];
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
+    const SlotDev &s = slots[jb.x];
     if (step >= (int)s.count) return;
     const ShapeDev &sh = *s.shp;
     constexpr int G_CT = RC ? g_for_R(RC) : 0;
@@ -747,8 +774,15 @@ __host__ __device__ constexpr bool io_two(int n) {
 __host__ __device__ constexpr int io_nstages(int n) { return io_two(n) ? 2 : ct_nstages(n); }
 __host__ __device__ constexpr int io_radix(int n, int idx) { return io_two(n) ? (idx == 0 ? n / 8 : 8) : ct_radix(n, idx); }
 
-template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, class LD, class ST>
-__device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST &st) {
+struct NoPre {
+    __device__ void operator()() const {}
+};
+
+// pre() runs once, after the first stage's LDS writes and before its barrier: a kernel stores
+// register-held tables to LDS there, after the first stage's HBM loads have been issued.
+template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, class LD, class ST,
+          class PRE = NoPre>
+__device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST &st, PRE pre = PRE()) {
     constexpr int NST = io_nstages(N);
     if constexpr (I < NST) {
         constexpr int P = io_radix(N, I), NB = N / P, TOTAL = NB * HOW;
@@ -797,6 +831,7 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
                     for (int r = 0; r < P; r++) a[(orow[it] + r * NS) * IS + seqs[it]] = v[it][r];
                 }
             }
+            if constexpr (I == 0) pre();
             __syncthreads();
             fft_io<N, NS * P, I + 1, HOW, IS, INV, LDIN, STOUT>(a, tw, ld, st);
         }
@@ -813,22 +848,21 @@ template <int N, int HOW> struct LastStage {
 // PPW_T pairs per work-group: ppw_for_C(C) fills every stage's threads; 1 spreads one test's
 // rows over more work-groups (lower latency when a single test runs alone).
 template <int C, int PPW_T = ppw_for_C(C)>
-__device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+__device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                           int step) {
     extern __shared__ double2 lds[];
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
+    const SlotDev &s = slots[jb.x];
     if (step >= (int)s.count) return;
-    const ShapeDev &sh = *s.shp;
     constexpr int PPW = PPW_T, NROW = 2 * PPW, NE = NROW * C;
     constexpr bool TWL = C <= kTwLdsMaxC;
     constexpr int PD = kRowPad;  // row buffer index i lives at i + i/8
-    const int R = sh.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
+    const int R = s.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
     double2 *A = lds;
-    const double2 *twC = sh.twC;
+    const double2 *twC = s.twC;
     if (TWL) {
         double2 *T = lds + padi<PD>(NE);
-        for (int i = threadIdx.x; i < C; i += kNT) T[i] = sh.twC[i];
+        for (int i = threadIdx.x; i < C; i += kNT) T[i] = gld(s.twC + i);
         twC = T;
     }
     auto row_of = [&](int i, int h) -> int {
@@ -841,7 +875,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
     for (int e = threadIdx.x; e < NE; e += kNT) {
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
-        A[padi<PD>(e)] = row >= 0 ? x[(size_t)row * C + c] : make_double2(0, 0);
+        A[padi<PD>(e)] = row >= 0 ? gld(x + (size_t)row * C + c) : make_double2(0, 0);
     }
     // split items: one per (local pair, k_c); the twiddle of item kc of pair p > 0 (and of row 0 of
     // pair 0) is fetched before the FFT
@@ -852,7 +886,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
     for (int it = 0; it < ITEMS; it++) {
         const int w = it * kNT + (int)threadIdx.x;
         const int i = w / C, kc = w - i * C, p = jb.y * PPW + i;
-        tw_[it] = (split && (NIT % kNT == 0 || w < NIT) && p < npairs) ? sh.twRows[(size_t)p * C + kc]
+        tw_[it] = (split && (NIT % kNT == 0 || w < NIT) && p < npairs) ? gld(s.twRows + (size_t)p * C + kc)
                                                                           : make_double2(1, 0);
     }
     __syncthreads();
@@ -861,7 +895,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
         for (int e = threadIdx.x; e < NE; e += kNT) {
             const int slot = e / C, c = e - slot * C;
             const int row = row_of(slot >> 1, slot & 1);
-            if (row >= 0) s.yfreq[(size_t)row * C + c] = A[padi<PD>(e)];
+            if (row >= 0) gst(s.yfreq + (size_t)row * C + c, A[padi<PD>(e)]);
         }
         return;
     }
@@ -876,7 +910,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
             const int pb = C - 1 - kc;
             const int ea = padi<PD>(2 * i * C + kc), eb = padi<PD>((2 * i + 1) * C + pb);
             double2 a = A[ea], b = A[eb];
-            if (mul) split_mul(a, b, tw_[it], s.yfreq[(size_t)p * C + kc], s.yfreq[(size_t)(R - p) * C + pb]);
+            if (mul) split_mul(a, b, tw_[it], gld(s.yfreq + (size_t)p * C + kc), gld(s.yfreq + (size_t)(R - p) * C + pb));
             else split_sq(a, b, tw_[it]);
             A[ea] = a;
             A[eb] = b;
@@ -885,7 +919,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
             int pb = (C - kc) % C;
             if (pb >= kc) {
                 double2 a = A[padi<PD>(kc)], b = A[padi<PD>(pb)];
-                if (mul) split_mul(a, b, tw_[it], s.yfreq[kc], s.yfreq[pb]);
+                if (mul) split_mul(a, b, tw_[it], gld(s.yfreq + kc), gld(s.yfreq + pb));
                 else split_sq(a, b, tw_[it]);
                 A[padi<PD>(kc)] = a;
                 if (pb != kc) A[padi<PD>(pb)] = b;
@@ -893,9 +927,9 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
             const int rh = row_of(0, 1);
             pb = C - 1 - kc;
             if (rh >= 0 && pb >= kc) {
-                const double2 t = sh.twRows[(size_t)rh * C + kc];
+                const double2 t = gld(s.twRows + (size_t)rh * C + kc);
                 double2 a = A[padi<PD>(C + kc)], b = A[padi<PD>(C + pb)];
-                if (mul) split_mul(a, b, t, s.yfreq[(size_t)rh * C + kc], s.yfreq[(size_t)rh * C + pb]);
+                if (mul) split_mul(a, b, t, gld(s.yfreq + (size_t)rh * C + kc), gld(s.yfreq + (size_t)rh * C + pb));
                 else split_sq(a, b, t);
                 A[padi<PD>(C + kc)] = a;
                 if (pb != kc) A[padi<PD>(C + pb)] = b;
@@ -907,7 +941,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
     for (int e = threadIdx.x; e < NE; e += kNT) {
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
-        if (row >= 0) s.x[(size_t)row * C + c] = A[padi<PD>(e)];
+        if (row >= 0) gst(s.x + (size_t)row * C + c, A[padi<PD>(e)]);
     }
 }
 
@@ -924,19 +958,6 @@ __device__ __forceinline__ void rows_fast(const SlotDev *slots, const SlotDev &o
 // and computes its own half of each pair. The inverse runs the forward steps backwards.
 constexpr int kRowsRegC = 256;
 
-// Loads and stores of device memory through global-address-space pointers: global (not flat)
-// instructions, which wait on the memory counter only.
-typedef double dv2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ double2 gld(const double2 *p) {
-    const dv2_t v = *(const __attribute__((address_space(1))) dv2_t *)p;
-    return make_double2(v.x, v.y);
-}
-__device__ __forceinline__ void gst(double2 *p, double2 v) {
-    dv2_t w;
-    w.x = v.x;
-    w.y = v.y;
-    *(__attribute__((address_space(1))) dv2_t *)p = w;
-}
 
 // The half of the real-split square (split_sq) that belongs to `u`: u is side a (partner w is
 // b) or, with isb, side b; tw = w_H^k of side a's frequency k.
@@ -1003,16 +1024,22 @@ __device__ __forceinline__ void xpose_ri(double2 *v, double *X, WR wr, RD rd) {
     __syncthreads();
 }
 
-__device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int step) {
+__device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int step) {
     constexpr int C = kRowsRegC;
     __shared__ double xs[8 * 256];  // one 2 KB block per 16-lane group
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
-    if (step >= (int)s.count) return;
-    const ShapeDev &sh = *s.shp;
+    const SlotDev &s = slots[jb.x];
+    {
+        // the slot fields of the prologue in one scalar round trip
+        const uint32_t count = s.count;
+        const int R0 = s.R;
+        const void *p0 = s.x, *p1 = s.tw16, *p2 = s.twRows;
+        asm volatile("" ::"s"(count), "s"(R0), "s"(p0), "s"(p1), "s"(p2));
+        if (step >= (int)count) return;
+    }
     const int lane = (int)threadIdx.x, g = lane >> 4, t = lane & 15;
     const int side = g >> 2, p = jb.y * 4 + (g & 3);
-    const int R = sh.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
+    const int R = s.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
     const bool special = p == 0;  // rows 0 and R/2, each paired with itself
     int row = -1;
     if (p < npairs) row = side == 0 ? p : (special ? (R % 2 == 0 ? R / 2 : -1) : R - p);
@@ -1021,11 +1048,16 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &
     double *X = xs + g * 256;
     double2 *xr = s.x + (size_t)(act ? row : 0) * C + t;
     // the row twiddles w_256^{t j} ([j][t], 4 KB) in LDS: L2 round trips in the FFTs cost more
+    // (its loads are issued first and stored to LDS after the row's loads are issued, so the row
+    // does not wait for the table's round trip)
     __shared__ double2 tws[256];
-    for (int i = lane; i < 256; i += (int)blockDim.x) tws[i] = sh.tw16[i];
+    static_assert(kRowsSqThreads * 2 == 256, "two twiddles per thread");
+    const double2 tw_a = gld(s.tw16 + lane), tw_b = gld(s.tw16 + lane + kRowsSqThreads);
     double2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = act ? gld(xr + 16 * i) : make_double2(0, 0);
+    tws[lane] = tw_a;
+    tws[lane + kRowsSqThreads] = tw_b;
     __syncthreads();
     const double2 *tw16 = tws;
     // forward: DFT16 over a, twiddle w_256^{t k1}, transpose, DFT16 over b
@@ -1047,7 +1079,7 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &
     double2 *Xc = reinterpret_cast<double2 *>(X);
     const bool first = jb.y == 0;  // uniform over the work-group: holds rows 0 and R/2
     const double2 *blk = (first && special) ? Xc : reinterpret_cast<const double2 *>(xs + (g ^ 4) * 256);
-    const double2 T0 = (act && !first) ? gld(sh.twRows + (size_t)p * C + t) : make_double2(0, 0);
+    const double2 T0 = (act && !first) ? gld(s.twRows + (size_t)p * C + t) : make_double2(0, 0);
     double2 lo[8];
 #pragma unroll
     for (int k2 = 8; k2 < 16; k2++) Xc[(km + 16 * k2) & 127] = v[k2];
@@ -1056,7 +1088,7 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &
         if (side == 0) rows256_split_half<0, 0, false>(v, lo, blk, km, T0);
         else rows256_split_half<0, 0, true>(v, lo, blk, km, T0);
     } else if (act) {
-        rows256_split_half0<0>(v, lo, blk, km, special, side, row, p, sh.twRows);
+        rows256_split_half0<0>(v, lo, blk, km, special, side, row, p, s.twRows);
     }
     __syncthreads();
 #pragma unroll
@@ -1066,7 +1098,7 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *slots, const SlotDev &
         if (side == 0) rows256_split_half<8, 8, false>(v, v + 8, blk, km, T0);
         else rows256_split_half<8, 8, true>(v, v + 8, blk, km, T0);
     } else if (act) {
-        rows256_split_half0<8>(v, v + 8, blk, km, special, side, row, p, sh.twRows);
+        rows256_split_half0<8>(v, v + 8, blk, km, special, side, row, p, s.twRows);
     }
     __syncthreads();
 #pragma unroll
@@ -1095,24 +1127,23 @@ template <int R> struct ColGeom {
 };
 
 template <int R>
-__device__ __forceinline__ void cols_inv_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+__device__ __forceinline__ void cols_inv_fast(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                               int step) {
     extern __shared__ double2 lds[];
     using Gm = ColGeom<R>;
     constexpr int G = Gm::G, SUB = Gm::SUB, NSUB = Gm::NSUB, NE = Gm::NE, NSC = Gm::NSC, ITSC = Gm::ITSC;
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
+    const SlotDev &s = slots[jb.x];
     if (step >= (int)s.count) return;
-    const ShapeDev &sh = *s.shp;
-    const int C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    const int C = s.C, NG = C / G, g = jb.y, c0 = g * G;
     double2 *A = lds, *T = lds + NE;
     int64_t *SC = reinterpret_cast<int64_t *>(T + R);
     int32_t *FD = reinterpret_cast<int32_t *>(SC + NSC);
-    for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
-    const double2 *tg = sh.twCols + (size_t)g * NE;
+    for (int i = threadIdx.x; i < R; i += kNT) T[i] = gld(s.twR + i);
+    const double2 *tg = s.twCols + (size_t)g * NE;
     for (int i = threadIdx.x; i < NE; i += kNT) {
         const int kr = i / G, l = i - kr * G;
-        A[i] = cmulc(s.x[(size_t)kr * C + c0 + l], tg[i]);
+        A[i] = cmulc(gld(s.x + (size_t)kr * C + c0 + l), gld(tg + i));
     }
     __syncthreads();
     fft_ip<R, 1, 0, G, G, 1, true, true>(A, T);
@@ -1128,17 +1159,17 @@ __device__ __forceinline__ void cols_inv_fast(const SlotDev *slots, const SlotDe
         const int w = it * kNT + (int)threadIdx.x;
         if (NSC % kNT != 0 && w >= NSC) continue;
         const int r = w / NSUB, q = w - r * NSUB;
-        const double tap = s.ta[R + r] * sc0;
+        const double tap = ldg(s.ta + R + r) * sc0;
         double cy = 0;
         bg[it] = 0;
 #pragma unroll
         for (int l = 0; l < SUB; l++) {
             const int col = q * SUB + l;
             const double2 v = A[r * G + col];
-            const int cd = code[r * G + col];
+            const int cd = ldg(code + r * G + col);
             const int cb = 2 * (c0 + col);
-            const double x0 = v.x * (tap * s.tb[2 * C + cb] * pow2ni(cd & 3));
-            const double x1 = v.y * (tap * s.tb[2 * C + cb + 1] * pow2ni((cd >> 3) & 3));
+            const double x0 = v.x * (tap * ldg(s.tb + 2 * C + cb) * pow2ni(cd & 3));
+            const double x1 = v.y * (tap * ldg(s.tb + 2 * C + cb + 1) * pow2ni((cd >> 3) & 3));
             const double r0 = rint(x0), r1 = rint(x1);
             lmax = fmax(lmax, fmax(fabs(x0 - r0), fabs(x1 - r1)));
             const int b0f = (cd >> 2) & 1, b1f = (cd >> 5) & 1;
@@ -1183,11 +1214,11 @@ __device__ __forceinline__ void cols_inv_fast(const SlotDev *slots, const SlotDe
                 cy = bal((int64_t)d[it][i] + cy, s.b0 + (int)((bg[it] >> i) & 1), d[it][i]);
             }
         }
-        int4 *o4 = reinterpret_cast<int4 *>(out + r * 2 * G + 2 * q * SUB);
+        int32_t *o4 = out + r * 2 * G + 2 * q * SUB;
 #pragma unroll
         for (int i = 0; i < 2 * SUB / 4; i++)
-            o4[i] = make_int4(d[it][4 * i], d[it][4 * i + 1], d[it][4 * i + 2], d[it][4 * i + 3]);
-        if (sq && q == NSUB - 1) s.cout[g * R + r] = SC[w];
+            gst4(o4 + 4 * i, make_int4(d[it][4 * i], d[it][4 * i + 1], d[it][4 * i + 2], d[it][4 * i + 3]));
+        if (sq && q == NSUB - 1) stg(s.cout + g * R + r, SC[w]);
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
     if ((threadIdx.x & 63) == 0 && lmax > 0)
@@ -1195,26 +1226,25 @@ __device__ __forceinline__ void cols_inv_fast(const SlotDev *slots, const SlotDe
 }
 
 template <int R>
-__device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs, int mode,
+__device__ __forceinline__ void cols_fwd_fast(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                               int step) {
     extern __shared__ double2 lds[];
     using Gm = ColGeom<R>;
     constexpr int G = Gm::G, SUB = Gm::SUB, NSUB = Gm::NSUB, NE = Gm::NE, NSC = Gm::NSC, ITSC = Gm::ITSC;
     constexpr int ITE = (NE + kNT - 1) / kNT;
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
+    const SlotDev &s = slots[jb.x];
     if (step >= (int)s.count) return;
-    const ShapeDev &sh = *s.shp;
-    const int C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    const int C = s.C, NG = C / G, g = jb.y, c0 = g * G;
     double2 *A = lds, *T = lds + NE;
-    for (int i = threadIdx.x; i < R; i += kNT) T[i] = sh.twR[i];
+    for (int i = threadIdx.x; i < R; i += kNT) T[i] = gld(s.twR + i);
     // four-step twiddles for the store, fetched early
     double2 tws[ITE];
 #pragma unroll
     for (int it = 0; it < ITE; it++) {
         const int i = it * kNT + (int)threadIdx.x;
         if (NE % kNT != 0 && i >= NE) continue;
-        tws[it] = sh.twCols[(size_t)g * NE + i];
+        tws[it] = gld(s.twCols + (size_t)g * NE + i);
     }
     const int32_t *in = s.dig + (size_t)g * R * 2 * G;
     const uint8_t *code = s.dcode + (size_t)g * R * G;
@@ -1225,10 +1255,10 @@ __device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDe
         if (NSC % kNT != 0 && w >= NSC) continue;
         const int r = w / NSUB, q = w - r * NSUB;
         int32_t d[2 * SUB];
-        const int4 *i4 = reinterpret_cast<const int4 *>(in + r * 2 * G + 2 * q * SUB);
+        const int32_t *i4 = in + r * 2 * G + 2 * q * SUB;
 #pragma unroll
         for (int i = 0; i < 2 * SUB / 4; i++) {
-            const int4 v = i4[i];
+            const int4 v = gld4(i4 + 4 * i);
             d[4 * i] = v.x;
             d[4 * i + 1] = v.y;
             d[4 * i + 2] = v.z;
@@ -1236,13 +1266,13 @@ __device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDe
         }
         int cds[SUB];
 #pragma unroll
-        for (int l = 0; l < SUB; l++) cds[l] = code[r * G + q * SUB + l];
+        for (int l = 0; l < SUB; l++) cds[l] = ldg(code + r * G + q * SUB + l);
         if (sq && q == 0) {  // the previous row chunk's carry, absorbed within this sub-chunk
             int pidx;
             if (g > 0) pidx = (g - 1) * R + r;
             else if (r > 0) pidx = (NG - 1) * R + r - 1;
             else pidx = (NG - 1) * R + R - 1;
-            int64_t cy = s.cout[pidx];
+            int64_t cy = ldg(s.cout + pidx);
 #pragma unroll
             for (int i = 0; i < 2 * SUB; i++) {
                 if (!cy) break;
@@ -1254,12 +1284,12 @@ __device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDe
                 cy = bal((int64_t)d[i] + cy, s.b0 + big, d[i]);
             }
         }
-        const double tam = s.ta[r];
+        const double tam = ldg(s.ta + r);
 #pragma unroll
         for (int l = 0; l < SUB; l++) {
             const int cb = 2 * (c0 + q * SUB + l);
-            const double w0 = tam * s.tb[cb] * pow2i(cds[l] & 3);
-            const double w1 = tam * s.tb[cb + 1] * pow2i((cds[l] >> 3) & 3);
+            const double w0 = tam * ldg(s.tb + cb) * pow2i(cds[l] & 3);
+            const double w1 = tam * ldg(s.tb + cb + 1) * pow2i((cds[l] >> 3) & 3);
             A[r * G + q * SUB + l] = make_double2((double)d[2 * l] * w0, (double)d[2 * l + 1] * w1);
         }
     }
@@ -1270,30 +1300,44 @@ __device__ __forceinline__ void cols_fwd_fast(const SlotDev *slots, const SlotDe
         const int i = it * kNT + (int)threadIdx.x;
         if (NE % kNT != 0 && i >= NE) continue;
         const int kr = i / G, l = i - kr * G;
-        s.x[(size_t)kr * C + c0 + l] = cmul(A[i], tws[it]);
+        gst(s.x + (size_t)kr * C + c0 + l, cmul(A[i], tws[it]));
     }
 }
 
 // Fused column pass (square mode): cols_inv_fast and cols_fwd_fast in one work-group, the digits
 // kept in registers. The carry out of each row chunk (g, r) goes to the work-group of the next
-// chunk through flags[g R + r] = seq << 40 | carry (low 40 bits, two's complement), written and
+// chunk through flags[g R + r] = seq << 48 | carry (low 48 bits, two's complement), written and
 // polled with agent-scope atomics: a single 64-bit word holds both, so no fence is needed. The
 // work-group of chunk (g, r) waits for chunk (g-1, r) (g > 0) or (NG-1, r-1) (g = 0; (NG-1, R-1)
 // holds the fold); every work-group publishes before it waits, and work-groups are dispatched in
 // order, so every wait ends. A wait is bounded: past kSpinLimit polls the slot's round-off is set
 // to infinity (the host reports the failure) and the carry is taken as 0.
-constexpr uint32_t kSeqMask = 0xFFFFFF;
+//
+// seq only has to differ from what the word held before this launch (the previous launch of the
+// same engine wrote seq - 1), so 16 bits suffice. A carry is below 2^52 / 2^(b0 - 1) in magnitude
+// (the digits before the carry are exact FP64 integers); the host keeps b0 >= 6 on this path, so
+// |carry| < 2^47 fits the 48-bit field. A carry that does not (only possible if a digit left the
+// exact range, which the round-off check reports anyway) also sets the round-off to infinity.
+constexpr uint32_t kSeqMask = 0xFFFF;
+constexpr int kFlagShift = 48;
+constexpr int64_t kFlagCarryMax = ((int64_t)1 << 47) - 1;
 constexpr int kSpinLimit = 1 << 22;
 
 __device__ __forceinline__ unsigned long long pack_flag(uint32_t seq, int64_t carry) {
-    return ((unsigned long long)(seq & kSeqMask) << 40) | ((unsigned long long)carry & ((1ull << 40) - 1));
+    return ((unsigned long long)(seq & kSeqMask) << kFlagShift) |
+           ((unsigned long long)carry & ((1ull << kFlagShift) - 1));
+}
+
+__device__ __forceinline__ bool flag_carry_fits(int64_t carry) {
+    return carry <= kFlagCarryMax && carry >= -kFlagCarryMax;
 }
 
 __device__ __forceinline__ int64_t wait_flag(unsigned long long *f, uint32_t seq, unsigned long long *maxerr) {
     const unsigned long long want = (unsigned long long)(seq & kSeqMask);
     for (int spin = 0;; spin++) {
         const unsigned long long v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((v >> 40) == want) return (int64_t)(v << 24) >> 24;  // sign-extend the 40-bit carry
+        if ((v >> kFlagShift) == want)
+            return (int64_t)(v << (64 - kFlagShift)) >> (64 - kFlagShift);  // sign-extend the carry
         if (spin >= kSpinLimit) {
             atomicMax(maxerr, 0x7FF0000000000000ull);  // +inf
             return 0;
@@ -1322,17 +1366,21 @@ template <int R> struct FusedGeom {
 };
 
 template <int R>
-__device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const SlotDev &one, const int2 *jobs,
-                                                int step) {
+__device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int step) {
     extern __shared__ double2 lds[];
     using Gm = FusedGeom<R>;
     constexpr int G = Gm::G, GP = Gm::GP, SUB = Gm::SUB, NSUB = Gm::NSUB, NE = Gm::NE, NSC = Gm::NSC;
     constexpr int ITSC = Gm::ITSC, ITE = Gm::ITE;
-    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    const SlotDev &s = pick_slot(slots, one, jb.x);
-    if (step >= (int)s.count) return;
-    const ShapeDev &sh = *s.shp;
-    const int C = sh.C, NG = C / G, g = jb.y, c0 = g * G;
+    {
+        // the slot fields of the prologue in one scalar round trip: issued together here, not one
+        // by one after the step test
+        const uint32_t count = s.count;
+        const int C0 = s.C;
+        const void *p0 = s.x, *p1 = s.twR, *p2 = s.twCols, *p3 = s.ta, *p4 = s.tb, *p5 = s.dcode;
+        asm volatile("" ::"s"(count), "s"(C0), "s"(p0), "s"(p1), "s"(p2), "s"(p3), "s"(p4), "s"(p5));
+        if (step >= (int)count) return;
+    }
+    const int C = s.C, NG = C / G, c0 = g * G;
     const uint32_t seq = s.seq + (uint32_t)step + 1;
     double2 *A = lds, *T = lds + R * GP;
     int64_t *SC = reinterpret_cast<int64_t *>(T + R);
@@ -1340,27 +1388,69 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
     double *WA = WB + 4 * G;                             // [R] inverse row (with k/(4H)), [R] forward row
     int32_t *FD = reinterpret_cast<int32_t *>(WA + 2 * R);
     uint8_t *CD = reinterpret_cast<uint8_t *>(FD + 64);  // [R][G] digit codes
-    for (int i = threadIdx.x; i < R; i += kNT) {
-        T[i] = sh.twR[i];
-        WA[i] = s.ta[R + i] * s.kscale;
-        WA[R + i] = s.ta[i];
+    // The group's tables (column twiddles, weight factors, digit codes) are loaded into registers
+    // here and stored to LDS by pre() below, once the first FFT stage has issued its HBM loads:
+    // their round trips overlap instead of preceding the residue's.
+    constexpr int NTR = (R + kNT - 1) / kNT, NTB = (2 * G + kNT - 1) / kNT, NTC = (R * G / 4 + kNT - 1) / kNT;
+    double2 rT[NTR];
+    double rWA[NTR], rWA2[NTR], rWB[NTB], rWB2[NTB];
+    uint32_t rCD[NTC];
+    const uint32_t *code4 = reinterpret_cast<const uint32_t *>(s.dcode + (size_t)g * R * G);
+#pragma unroll
+    for (int q = 0; q < NTR; q++) {
+        const int i = q * kNT + (int)threadIdx.x;
+        if (R % kNT == 0 || i < R) {
+            rT[q] = gld(s.twR + i);
+            rWA[q] = ldg(s.ta + R + i);
+            rWA2[q] = ldg(s.ta + i);
+        }
     }
-    for (int i = threadIdx.x; i < 2 * G; i += kNT) {
-        WB[i] = s.tb[2 * C + 2 * c0 + i];
-        WB[2 * G + i] = s.tb[2 * c0 + i];
+#pragma unroll
+    for (int q = 0; q < NTB; q++) {
+        const int i = q * kNT + (int)threadIdx.x;
+        if ((2 * G) % kNT == 0 || i < 2 * G) {
+            rWB[q] = ldg(s.tb + 2 * C + 2 * c0 + i);
+            rWB2[q] = ldg(s.tb + 2 * c0 + i);
+        }
     }
-    {
-        const uint32_t *code4 = reinterpret_cast<const uint32_t *>(s.dcode + (size_t)g * R * G);
-        for (int i = threadIdx.x; i < R * G / 4; i += kNT) reinterpret_cast<uint32_t *>(CD)[i] = code4[i];
+#pragma unroll
+    for (int q = 0; q < NTC; q++) {
+        const int i = q * kNT + (int)threadIdx.x;
+        if ((R * G / 4) % kNT == 0 || i < R * G / 4) rCD[q] = ldg(code4 + i);
     }
-    const double2 *tg = sh.twCols + (size_t)g * NE;
+    const double kscale = s.kscale;
+    auto pre = [&]() {
+#pragma unroll
+        for (int q = 0; q < NTR; q++) {
+            const int i = q * kNT + (int)threadIdx.x;
+            if (R % kNT == 0 || i < R) {
+                T[i] = rT[q];
+                WA[i] = rWA[q] * kscale;
+                WA[R + i] = rWA2[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NTB; q++) {
+            const int i = q * kNT + (int)threadIdx.x;
+            if ((2 * G) % kNT == 0 || i < 2 * G) {
+                WB[i] = rWB[q];
+                WB[2 * G + i] = rWB2[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NTC; q++) {
+            const int i = q * kNT + (int)threadIdx.x;
+            if ((R * G / 4) % kNT == 0 || i < R * G / 4) reinterpret_cast<uint32_t *>(CD)[i] = rCD[q];
+        }
+    };
+    const double2 *tg = s.twCols + (size_t)g * NE;
     // inverse column FFT, its first stage reading the residue (times the conjugate four-step
     // twiddle) straight from HBM; the tables above are published by its first barrier
     {
         const double2 *xin = s.x + c0;
         auto ld = [&](int, int, int row, int col) { return cmulc(gld(xin + (size_t)row * C + col), gld(tg + row * G + col)); };
         auto st = [](int, int, int, int, double2) {};
-        fft_io<R, 1, 0, G, GP, true, true, false>(A, T, ld, st);
+        fft_io<R, 1, 0, G, GP, true, true, false>(A, T, ld, st, pre);
     }
     const int b0 = s.b0;
     double lmax = 0;
@@ -1402,8 +1492,10 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
             for (int i = 0; i < 2 * SUB; i++) FD[i] = d[it][i];
         }
         SC[w] = co;
-        if (q == NSUB - 1 && !folder)  // this row chunk's carry, for the next chunk's work-group
+        if (q == NSUB - 1 && !folder) {  // this row chunk's carry, for the next chunk's work-group
+            if (!flag_carry_fits(co)) atomicMax(s.maxerr, 0x7FF0000000000000ull);
             __hip_atomic_store(s.flags + g * R + r, pack_flag(seq, co), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     __syncthreads();
     // the top fold, here where few registers are live (its 128-bit arithmetic inside the loop above
@@ -1412,6 +1504,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev *slots, const Slot
         constexpr int it = ITSC - 1;
         const int64_t co = fold_top_dev(s, FD, SC[NSC - 1]);
         SC[NSC - 1] = co;
+        if (!flag_carry_fits(co)) atomicMax(s.maxerr, 0x7FF0000000000000ull);
         __hip_atomic_store(s.flags + g * R + R - 1, pack_flag(seq, co), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int i = 0; i < 2 * SUB; i++) d[it][i] = FD[i];

@@ -9,7 +9,7 @@
 
 namespace kite {
 
-typedef const void *PassKernel;  // a __global__ function (const SlotDev*, SlotDev, const int2*, int, int)
+typedef const void *PassKernel;  // a __global__ function (const SlotDev *slots, const int2 *jobs, int mode, int step)
 
 PassKernel generic_rows();
 PassKernel generic_cols_inv();
@@ -21,7 +21,6 @@ PassKernel fast_rows1(int C);
 // square-mode row kernel with the rows held in registers (128 threads per work-group, four row
 // pairs like the LDS row kernel, static LDS only); nullptr for row lengths without one
 PassKernel fast_rows_sq(int C);
-constexpr int kRowsSqThreads = 128;
 PassKernel fast_cols_inv(int R);
 PassKernel fast_cols_fwd(int R);
 size_t fast_cols_lds(int R);  // LDS bytes of the compile-time column kernels

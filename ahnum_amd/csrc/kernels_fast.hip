@@ -7,55 +7,57 @@
 namespace kite {
 
 template <int C>
-__global__ void __launch_bounds__(kNT) k_rows_c(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
-    rows_fast<C>(slots, one, jobs, mode, step);
+__global__ void __launch_bounds__(kNT) k_rows_c(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
+    rows_fast<C>(slots, jobs, mode, step);
 }
 // One row pair per work-group: a single test's row pass on four times as many work-groups.
 template <int C>
-__global__ void __launch_bounds__(kNT) k_rows_c1(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
-    rows_fast<C, 1>(slots, one, jobs, mode, step);
+__global__ void __launch_bounds__(kNT) k_rows_c1(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
+    rows_fast<C, 1>(slots, jobs, mode, step);
 }
-__global__ void __launch_bounds__(128, 3) k_rows_sq256(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+__global__ void __launch_bounds__(128, 3) k_rows_sq256(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
     (void)mode;
-    rows256_sq(slots, one, jobs, step);
+    rows256_sq(slots, jobs, step);
 }
 template <int R>
-__global__ void __launch_bounds__(kNT) k_cols_inv_r(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
-    cols_inv_fast<R>(slots, one, jobs, mode, step);
+__global__ void __launch_bounds__(kNT) k_cols_inv_r(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
+    cols_inv_fast<R>(slots, jobs, mode, step);
 }
 template <int R>
-__global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
-    cols_fwd_fast<R>(slots, one, jobs, mode, step);
+__global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
+    cols_fwd_fast<R>(slots, jobs, mode, step);
 }
 
 template <int R>
-__global__ void __launch_bounds__(kNT, 4) k_cols_fused_r(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_r(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
     (void)mode;
-    cols_fused_fast<R>(slots, one, jobs, step);
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    cols_fused_fast<R>(slots[jb.x], jb.y, step);
 }
 
 // Two adjacent column lengths in one launch: tests of both shapes advance together (each work-
 // group runs the body of its test's R), so a batch straddling an FFT-length boundary fills the
 // device with one launch per column pass instead of two.
 template <int R1, int R2>
-__global__ void __launch_bounds__(kNT) k_cols_inv_rr(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+__global__ void __launch_bounds__(kNT) k_cols_inv_rr(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    if (pick_slot(slots, one, jb.x).shp->R == R1) cols_inv_fast<R1>(slots, one, jobs, mode, step);
-    else cols_inv_fast<R2>(slots, one, jobs, mode, step);
+    if (slots[jb.x].R == R1) cols_inv_fast<R1>(slots, jobs, mode, step);
+    else cols_inv_fast<R2>(slots, jobs, mode, step);
 }
 template <int R1, int R2>
-__global__ void __launch_bounds__(kNT) k_cols_fwd_rr(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+__global__ void __launch_bounds__(kNT) k_cols_fwd_rr(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    if (pick_slot(slots, one, jb.x).shp->R == R1) cols_fwd_fast<R1>(slots, one, jobs, mode, step);
-    else cols_fwd_fast<R2>(slots, one, jobs, mode, step);
+    if (slots[jb.x].R == R1) cols_fwd_fast<R1>(slots, jobs, mode, step);
+    else cols_fwd_fast<R2>(slots, jobs, mode, step);
 }
 
 template <int R1, int R2>
-__global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode, int step) {
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode, int step) {
     (void)mode;
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    if (pick_slot(slots, one, jb.x).shp->R == R1) cols_fused_fast<R1>(slots, one, jobs, step);
-    else cols_fused_fast<R2>(slots, one, jobs, step);
+    const SlotDev &s = slots[jb.x];
+    if (s.R == R1) cols_fused_fast<R1>(s, jb.y, step);
+    else cols_fused_fast<R2>(s, jb.y, step);
 }
 
 size_t fast_cols_fused_lds(int R) {

@@ -6,17 +6,17 @@
 
 namespace kite {
 
-__global__ void __launch_bounds__(kNT) k_rows_g(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode,
+__global__ void __launch_bounds__(kNT) k_rows_g(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                               int step) {
-    rows_body<0>(slots, one, jobs, mode, step);
+    rows_body<0>(slots, jobs, mode, step);
 }
-__global__ void __launch_bounds__(kNT) k_cols_inv_g(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode,
+__global__ void __launch_bounds__(kNT) k_cols_inv_g(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                                   int step) {
-    cols_inv_body<0>(slots, one, jobs, mode, step);
+    cols_inv_body<0>(slots, jobs, mode, step);
 }
-__global__ void __launch_bounds__(kNT) k_cols_fwd_g(const SlotDev *slots, SlotDev one, const int2 *jobs, int mode,
+__global__ void __launch_bounds__(kNT) k_cols_fwd_g(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int mode,
                                                   int step) {
-    cols_fwd_body<0>(slots, one, jobs, mode, step);
+    cols_fwd_body<0>(slots, jobs, mode, step);
 }
 
 PassKernel generic_rows() { return (PassKernel)k_rows_g; }

@@ -79,6 +79,7 @@ void engine_addsub(kite_ctx *ctx, void *x, const void *y, int sign);
 double engine_maxerr(kite_ctx *ctx);
 void engine_reset_maxerr(kite_ctx *ctx);
 void engine_sync(kite_ctx *ctx);
+int engine_error(const kite_ctx *ctx);  // 0, or -1 once a HIP call of this context has failed
 const char *engine_last_error();
 
 // ---- LLR helpers (llr.cpp) -------------------------------------------------------------------
