@@ -776,6 +776,9 @@ struct kite_batch {
     hipGraphExec_t gexec = nullptr;
     uint32_t gsteps = 0;
     uint64_t ggen = 0;
+    // column jobs are laid out for this many XCDs (work-group i runs on XCD i mod xcds): each
+    // test's column groups go to one XCD, in order (1: plain order)
+    int xcds = kite::kXcds;
     int timing = 0;  // 0: off; N: every N-th step of a square call is timed
     // timing: one event pair per launch of the last timed call, and what each pair measured
     std::vector<hipEvent_t> ev;
@@ -892,6 +895,7 @@ static int batch_refresh(kite_batch *b) {
             std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
                 return b->slots[x].ctx->eng->shape->ch.R < b->slots[y].ctx->eng->shape->ch.R;
             });
+        // per-slot job lists of this part, so that a group's column jobs can be interleaved
         for (int s0 : ord) {
             if (done[s0]) continue;
             Engine *E0 = b->slots[s0].ctx->eng;
@@ -934,6 +938,32 @@ static int batch_refresh(kite_batch *b) {
             out.insert(out.end(), grp.begin(), grp.end());
             return;
         }
+        // Work-group i of a launch runs on XCD i mod xcds. The jobs of each column group are
+        // reordered so that every test's column groups (whose carries pass between neighbours
+        // through flags) land on one XCD, consecutively: neighbours then start together and
+        // exchange their flags through that XCD's L2. Each test's jobs keep their relative order,
+        // so every flag a work-group waits for is still published by one dispatched before it or
+        // by its own test's last group, as before.
+        if (b->xcds > 1)
+            for (const kite_batch::Group &g : grp) {
+                std::vector<std::vector<int2>> lists(b->xcds);
+                int t = -1, prev = -1;
+                for (int i = g.off; i < g.off + g.count; i++) {
+                    if (jobs[i].x != prev) {
+                        prev = jobs[i].x;
+                        t++;
+                    }
+                    lists[t % b->xcds].push_back(jobs[i]);
+                }
+                std::vector<size_t> pos(b->xcds, 0);
+                int i = g.off;
+                for (size_t q = 0; i < g.off + g.count; q++)
+                    for (int x = 0; x < b->xcds && i < g.off + g.count; x++) {
+                        int xx = x;  // a drained list lends its position to the next one with jobs
+                        while (pos[xx] >= lists[xx].size()) xx = (xx + 1) % b->xcds;
+                        jobs[i++] = lists[xx][pos[xx]++];
+                    }
+            }
         // adjacent compile-time column lengths share one launch per pass (the fused pass only if
         // both lengths have it)
         std::vector<kite_batch::Group> merged;
@@ -1296,6 +1326,15 @@ extern "C" int kite_batch_maxerr(kite_batch *b, double *out, int reset) {
         BATCH_OK(b, hipStreamSynchronize(b->stream));
     }
     return 0;
+}
+
+extern "C" int kite_batch_set_xcds(kite_batch *b, int xcds) {
+    xcds = std::max(1, std::min(xcds, 64));
+    if (xcds != b->xcds) {
+        b->xcds = xcds;
+        b->dirty = true;
+    }
+    return xcds;
 }
 
 extern "C" int kite_batch_set_graphs(kite_batch *b, int on) {

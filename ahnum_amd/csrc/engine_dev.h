@@ -17,6 +17,7 @@ constexpr int kMaxTop = 64;         // top-region digits handled by the device f
 constexpr int kSub = 8;             // elements (16 digits) per carry sub-chunk
 constexpr int kTwLdsMaxC = 1024;    // row twiddles are staged in LDS up to this row length
 constexpr int kRowPad = 3;          // row-kernel LDS: one padding element per 8
+constexpr int kXcds = 8;            // MI355X: 8 XCDs; work-group i of a launch runs on XCD i mod 8
 constexpr int kFusedMinBits = 6;    // fused column pass: digits of at least this many bits (carry < 2^47)
 constexpr int kRowsSqThreads = 128; // threads per work-group of the register row kernel (C = 256)
 

@@ -387,6 +387,7 @@ SIGNATURES.update({
     "kite_batch_set_xadd": (ctypes.c_int, [c_p, ctypes.c_int, ctypes.c_int32]),
     "kite_batch_set_graphs": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_set_parts": (ctypes.c_int, [c_p, ctypes.c_int]),
+    "kite_batch_set_xcds": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_job_bytes": (c_u64, [c_u32, c_u32]),
     "kite_batch_run": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(RunOpts),
                                       ctypes.POINTER(JobResult)]),
@@ -462,13 +463,15 @@ class Batch:
     def set(self, s, i, v):
         c = self.ctx(s)
         nl = self.lib.kite_limbs(c)
-        self.lib.kite_batch_sync(self.p)
+        if self.lib.kite_batch_sync(self.p):
+            raise RuntimeError("kite batch: a HIP call failed")
         self.lib.kite_set_limbs(c, self.lib.kite_batch_num(self.p, s, i), to_limbs(v, nl))
 
     def get(self, s, i):
         c = self.ctx(s)
         nl = self.lib.kite_limbs(c)
-        self.lib.kite_batch_sync(self.p)
+        if self.lib.kite_batch_sync(self.p):
+            raise RuntimeError("kite batch: a HIP call failed")
         arr = (c_u64 * nl)()
         self.lib.kite_get_limbs(c, self.lib.kite_batch_num(self.p, s, i), arr)
         return from_limbs(arr)
@@ -490,11 +493,16 @@ class Batch:
         return {s: out[s] for s in self.slots}
 
     def sync(self):
-        self.lib.kite_batch_sync(self.p)
+        if self.lib.kite_batch_sync(self.p):
+            raise RuntimeError("kite batch: a HIP call failed")
 
     def parts(self, n):
         """Deal the slots to n concurrent streams (1..4); returns the count in use."""
         return self.lib.kite_batch_set_parts(self.p, int(n))
+
+    def xcds(self, n):
+        """Lay column jobs out for n XCDs (each test's column groups on one XCD; 1: slot order)."""
+        return self.lib.kite_batch_set_xcds(self.p, int(n))
 
     def graphs(self, on):
         """Replay multi-step square calls from a captured HIP graph (the default) or launch directly."""

@@ -37,6 +37,10 @@ int kite_batch_square(kite_batch *b, const int32_t *which, const uint32_t *count
 // The slots are dealt to `parts` streams (1..4, default 2) whose launches run concurrently, so
 // kernels of different parts overlap their memory and compute phases. Returns the count in use.
 int kite_batch_set_parts(kite_batch *b, int parts);
+// Column jobs are laid out for `xcds` XCDs (work-group i of a launch runs on XCD i mod xcds; 8 on
+// MI355X, the default): each test's column groups run on one XCD. 1 keeps plain slot order.
+// Returns the value in use.
+int kite_batch_set_xcds(kite_batch *b, int xcds);
 // Square calls of more than one step are captured as a HIP graph and replayed while the batch's
 // slots are unchanged (on by default; the launches are the same either way).
 int kite_batch_set_graphs(kite_batch *b, int on);
@@ -60,6 +64,11 @@ int kite_batch_pass_times(kite_batch *b, double *out3, uint64_t *launches3);
 // column launch). Returns the number of kernels (i < 0 only queries it), -1 for i out of range.
 int kite_batch_kernel_times(kite_batch *b, int i, char *name, size_t name_len, double *ms,
                             uint64_t *launches, double *bytes);
+
+// Testing hook: the n-th HIP call the library checks from now on (in any context or batch) is
+// reported as failed (n <= 0: off). Nothing fails on the device; the library takes its error path:
+// the context's round-off reads +inf, batch calls return -1 and the drivers a negative code.
+void kite_debug_fail_hip(long n);
 
 // ---- batched Fermat PRP tests ---------------------------------------------------------------
 
