@@ -1115,12 +1115,9 @@ extern "C" void kite_batch_free(kite_batch *b) {
     delete b;
 }
 
-extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t fft_len, uint32_t nnums) {
+int kite::batch_add_opts(kite_batch *b, uint64_t k, uint64_t n, const kite_opts *o, uint32_t nnums) {
     if (b->err != hipSuccess) return -1;
-    kite_opts o{};
-    o.backend = KITE_BACKEND_HIP;
-    o.fft_len = fft_len;
-    kite_ctx *ctx = kite_ctx_new(k, n, -1, &o);
+    kite_ctx *ctx = kite_ctx_new(k, n, -1, o);
     if (!ctx) return -1;
     BatchSlot bs;
     bs.ctx = ctx;
@@ -1150,6 +1147,13 @@ extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t ff
     BATCH_OK(b, hipGetLastError());
     BATCH_OK(b, hipStreamSynchronize(b->stream));
     return (int)s;
+}
+
+extern "C" int kite_batch_add(kite_batch *b, uint64_t k, uint64_t n, uint32_t fft_len, uint32_t nnums) {
+    kite_opts o{};
+    o.backend = KITE_BACKEND_HIP;
+    o.fft_len = fft_len;
+    return batch_add_opts(b, k, n, &o, nnums);
 }
 
 extern "C" void kite_batch_remove(kite_batch *b, int slot) {

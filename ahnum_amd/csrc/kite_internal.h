@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kite.h"
+#include "kite_batch.h"
 
 namespace kite {
 struct Engine;  // device state of one context (engine.hip)
@@ -82,15 +83,16 @@ void engine_sync(kite_ctx *ctx);
 int engine_error(const kite_ctx *ctx);  // 0, or -1 once a HIP call of this context has failed
 const char *engine_last_error();
 
-// ---- LLR helpers (llr.cpp) -------------------------------------------------------------------
+// ---- LLR arithmetic (llr_math.cpp) -----------------------------------------------------------
 // x = V_k(P, -1) by the Lucas ladder (cP holds P, c2 holds 2; v1, t are scratch).
 void llr_lucas_vk(kite_ctx *ctx, uint64_t k, kite_num *x, kite_num *v1, kite_num *cP, kite_num *c2, kite_num *t);
 uint64_t n_mod(uint64_t k, uint64_t n, uint64_t m);  // (k 2^n - 1) mod m
-// x -> x^2 + xadd, count times (|xadd| < 2^20): the addend enters in the carry pass's top fold,
-// so an LLR step costs one squaring (api.cpp)
-void square_add(kite_ctx *ctx, kite_num *x, uint64_t count, int32_t xadd);
 
-// ---- checkpoint files (prp.cpp, llr.cpp) -----------------------------------------------------
+// ---- batch slots with context options (engine.hip) --------------------------------------------
+// kite_batch_add with the context options of a kite.h caller (backend, length, shape variant).
+int batch_add_opts(kite_batch *b, uint64_t k, uint64_t n, const kite_opts *o, uint32_t nnums);
+
+// ---- checkpoint files (ckpt.cpp) -------------------------------------------------------------
 inline size_t limbs_for_n(uint64_t n) { return (n + 64 + 63) / 64; }  // limbs of a residue, k < 2^64
 // KITECKP1: Fermat state (x, d) at iteration iter, written at length L; 0 on success. The reader
 // checks k, n, base, the limb count and the checksum.

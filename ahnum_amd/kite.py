@@ -359,6 +359,9 @@ class JobResult(ctypes.Structure):
         ("finish", c_dbl),
         ("resumed_at", c_u64),
         ("resumed", c_u32),
+        ("run", c_u32),
+        ("iter", c_u64),
+        ("verified", c_u64),
     ]
 
 

@@ -2,26 +2,32 @@
 """Benchmark of the MI355X squaring engine on the Fermat workload at n ~ 660K (FFT ~49152).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--cpu-secs S] [--no-cpu]
+                  [--gerbicz-secs S] [--serve-secs S]
   python bench.py --workload {batch_mixed,large_fft,record,plain_small} [...]
   python bench.py --serve [--serve-count C] [--serve-rate R] [--serve-horizon S] ...
 
 A step is one squaring of every residue in the batch (B Fermat base-3 tests with k odd in
-[1001, 9999], n in [650000, 670000], seeded; B = 384 by default, the batch size that measured
-fastest on one MI355X among 256-768): the batched engine advances all B tests with one launch
-per pass and stream. The same number is also timed alone through the kite.h path (the plain
-one-test-at-a-time GPU path). tests/hour follows the reference's throughput formula
-(tools/kite.c:262): sum over tests of squarings/s * 3600 / n.
+[1001, 9999], n in [650000, 670000], seeded; B = 384 by default): the batched engine advances all
+B tests with one launch per pass and stream. `value` is the squaring throughput of the K timed
+steps as tests/hour, the reference's throughput formula (tools/kite.c:262): sum over tests of
+squarings/s * 3600 / n. The K steps run in square calls of min(K, 1000) steps (a Gerbicz block),
+replayed from the HIP graph the batch captured for that call; the graph is captured and replayed
+once in full before the timed region, then W warm-up steps run.
 
-The timed region runs square calls of 1000 steps (a Gerbicz block), each replayed from the HIP
-graph the batch captured for it. Prints one JSON line. `roofline` is for the dominant pass kernel
-(the most device time), timed right after in a second region of 1000 steps launched directly, with
-HIP events on the batch stream around each launch of every 10th step;
-algorithmic bytes per launch are 16 L per test for the row pass and for the fused column pass
-(each reads and writes the 8L-byte residue; 32 L per squaring in total). `path_roofline` puts the
-whole step against both rooflines. `traffic` is the
-kernel's HBM bytes per launch from profiles/hbm_traffic_single_large_batch.txt (rocprofv3
-FETCH_SIZE and WRITE_SIZE runs of this benchmark, calibrated by a copy of known size). `cpu_baseline` is the oracle's restatement of the reference's portable
-two-pass backend (--backend c2p) timed on the host cores, one test per core.
+Beside it, one JSON line carries:
+  plain          the same number alone through kite.h (10^4 squarings of a seeded random residue):
+                 us/squaring and max round-off, the one-test-at-a-time GPU path
+  roofline       the dominant pass kernel (the most device time) in a second region of launches
+                 timed with HIP events on the batch stream: algorithmic bytes per launch (16 L per
+                 test: each pass reads and writes the 8L-byte residue) / average launch duration
+  path_roofline  the whole step against HBM (32 L bytes per test) and FP64 (5 L log2(L/2) + 20 L)
+  gerbicz_checked  the same tests through kite_batch_run, the production loop (Gerbicz blocks of
+                 1000 with their multiplies, checks every M blocks with limb conversions, round-off
+                 reads), for a bounded time: verified squarings/s as tests/hour
+  serve          a bounded serving run: Poisson arrivals of Fermat and LLR requests (n log-uniform
+                 in [200K, 5M]), p50/p99 completion latency per kind, backlog at the horizon
+  cpu_baseline   the oracle's restatement of the reference's portable two-pass backend (c2p),
+                 one test per host core, timed on this host
 """
 import argparse
 import json
@@ -34,7 +40,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# the batch workloads (ahnum_amd/workloads.py): metric, description, test count (None: --batch)
+# the batch workloads (ahnum_amd/workloads.py): metric, description, test count (None: --batch), seed
 WORKLOADS = {
     "single_large": ("tests/hour at n~660K (Fermat base 3, FFT 49152, Gerbicz-checkable residues), batched",
                      "single_large batch: B Fermat base-3 tests, k odd in [1001,9999], n in [650000,670000]",
@@ -51,6 +57,39 @@ WORKLOADS = {
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (FMA counted as two)
+
+
+# ---- CPU baseline ------------------------------------------------------------------------------
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_has_v3():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return {"avx2", "fma", "bmi2"} <= flags
+    except OSError:
+        pass
+    return False
+
+
+def host_cores():
+    """Cores this process may use: its CPU affinity, within the share the host grants the job
+    (OMP_NUM_THREADS, when the host sets it to announce that share)."""
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(share)) if share.isdigit() and int(share) > 0 else n
 
 
 def cpu_worker(args):
@@ -73,10 +112,16 @@ def cpu_worker(args):
 
 
 def cpu_baseline(tests, secs):
-    cores = min(os.cpu_count() or 1, 16)
+    cores = host_cores()
+    fast = os.path.join(ROOT, "oracle", "bin", "liboracle_v3.so")
+    isa = "x86-64-v2, -O2"
+    if cpu_has_v3() and os.path.exists(fast):
+        os.environ["KITE_ORACLE_LIB"] = fast  # inherited by the spawned workers
+        isa = "x86-64-v3, -O3"
     jobs = [(k, n, secs) for k, n in (tests * cores)[:cores]]
     with mp.get_context("spawn").Pool(cores) as pool:
         out = pool.map(cpu_worker, jobs)
+    os.environ.pop("KITE_ORACLE_LIB", None)
     tph = sum(rate * 3600.0 / n for (rate, _), (_, n, _) in zip(out, jobs))
     us = 1e6 / (sum(r for r, _ in out) / len(out))
     return {
@@ -84,10 +129,15 @@ def cpu_baseline(tests, secs):
         "unit": "tests/hour",
         "cores": cores,
         "kind": "port",
-        "sample": f"c2p restatement (oracle/kite_oracle.c), one test per core, {secs:.0f} s of squarings per "
-                  f"core at L={out[0][1]} ({us:.0f} us/squaring/core), extrapolated as rate*3600/n",
+        "cpu": cpu_model(),
+        "build": isa,
+        "sample": f"c2p restatement (oracle/kite_oracle.c, {isa}), one test per core on {cores} cores, "
+                  f"{secs:.0f} s of squarings per core at L={out[0][1]} ({us:.0f} us/squaring/core), "
+                  f"extrapolated as rate*3600/n",
     }
 
+
+# ---- traffic summary ---------------------------------------------------------------------------
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_large_batch.txt")
 
@@ -106,59 +156,111 @@ def load_traffic(kernel):
     return None
 
 
-def serve(a):
-    """Serving run: a finite Poisson stream of Fermat and LLR requests through kite_batch_run."""
+# ---- serving -----------------------------------------------------------------------------------
+
+def pct(values, p):
+    v = sorted(values)
+    return round(v[min(len(v) - 1, int(p * len(v)))], 3) if v else None
+
+
+def run_serve(count, rate, n_lo, n_hi, horizon, hbm_gb, slots, order_name, ckpt_dir="", ckpt_secs=0.0,
+              verbose=False):
+    """A finite Poisson stream of Fermat and LLR requests through kite_batch_run; the serving record."""
     from ahnum_amd import kite, workloads
-    stream = workloads.serve_stream(count=a.serve_count, rate=a.serve_rate, n_lo=a.serve_n_lo, n_hi=a.serve_n_hi)
+    stream = workloads.serve_stream(count=count, rate=rate, n_lo=n_lo, n_hi=n_hi)
     jobs = [(kite.JOB_LLR if kind == "llr" else kite.JOB_PRP, k, n, t) for t, kind, k, n in stream]
-    t0 = time.perf_counter()
+
     def progress(i, r):  # one line per completed request, on stderr
-        print(f"done {i} status {r.status} n {jobs[i][2]} latency {r.finish - jobs[i][3]:.2f}s", file=sys.stderr,
-              flush=True)
+        if verbose:
+            print(f"done {i} status {r.status} n {jobs[i][2]} latency {r.finish - jobs[i][3]:.2f}s",
+                  file=sys.stderr, flush=True)
 
-    if a.serve_checkpoint_dir:
-        os.makedirs(a.serve_checkpoint_dir, exist_ok=True)
-    order = kite.ORDER_LEAST_WORK if a.serve_order == "least_work" else kite.ORDER_ARRIVAL
-    rc, out = kite.batch_run(jobs, slots=a.serve_slots, hbm_budget=int(a.serve_hbm_gb * 2**30), block=1000,
-                             horizon=a.serve_horizon, on_done=progress, checkpoint_dir=a.serve_checkpoint_dir or None,
-                             checkpoint_secs=a.serve_checkpoint_secs, order=order)
+    if ckpt_dir:
+        os.makedirs(ckpt_dir, exist_ok=True)
+    order = kite.ORDER_LEAST_WORK if order_name == "least_work" else kite.ORDER_ARRIVAL
+    t0 = time.perf_counter()
+    rc, out = kite.batch_run(jobs, slots=slots, hbm_budget=int(hbm_gb * 2**30), block=1000, horizon=horizon,
+                             on_done=progress, checkpoint_dir=ckpt_dir or None, checkpoint_secs=ckpt_secs,
+                             order=order)
     wall = time.perf_counter() - t0
-    done = [(j, r) for j, r in zip(jobs, out) if r.status == 0]
-    lat = sorted(r.finish - j[3] for j, r in done)
+    span = min(wall, horizon) if horizon else wall
+    arrived = sum(1 for j in jobs if j[3] <= span)
+    factor = [(j, r) for j, r in zip(jobs, out) if r.status == 0 and j[0] == kite.JOB_LLR and r.factor]
+    tested = [(j, r) for j, r in zip(jobs, out) if r.status == 0 and not (j[0] == kite.JOB_LLR and r.factor)]
+    lat = {"fermat": [r.finish - j[3] for j, r in tested if j[0] == kite.JOB_PRP],
+           "llr": [r.finish - j[3] for j, r in tested if j[0] == kite.JOB_LLR]}
+    rec = {
+        "requests": count, "rate_per_s": rate, "n_range": [n_lo, n_hi], "horizon_s": horizon,
+        "hbm_budget_gib": hbm_gb, "slots": slots, "order": order_name, "block": 1000,
+        "arrived": arrived,
+        "completed": {"fermat": len(lat["fermat"]), "llr": len(lat["llr"])},
+        "completed_tests_per_hour": round(len(tested) * 3600.0 / span, 1),
+        "llr_factor_found": len(factor),
+        "latency_p50_s": {kind: pct(v, 0.50) for kind, v in lat.items()},
+        "latency_p99_s": {kind: pct(v, 0.99) for kind, v in lat.items()},
+        "max_completed_n": max((j[2] for j, _ in tested), default=0),
+        "in_flight_at_horizon": sum(1 for r in out if r.status == kite.JOB_PENDING and r.fft_len),
+        "waiting_at_horizon": sum(1 for j, r in zip(jobs, out) if r.status == kite.JOB_PENDING and not r.fft_len
+                                  and j[3] <= span),
+        "failed": sum(1 for r in out if r.status not in (0, kite.JOB_PENDING)),
+        "maxerr": round(max((r.maxerr for _, r in tested), default=0.0), 4),
+        "wall_s": round(wall, 2),
+    }
+    if ckpt_dir:  # restartable serving: the stopped jobs' states are on disk
+        rec["checkpoint_dir"] = ckpt_dir
+        rec["resumed"] = sum(1 for r in out if r.resumed)
+        rec["checkpoint_files"] = sum(1 for f in os.listdir(ckpt_dir) if f.endswith(".ckpt"))
+    return rec
 
-    def pct(p):
-        return round(lat[min(len(lat) - 1, int(p * len(lat)))], 3) if lat else None
-    horizon = min(wall, a.serve_horizon) if a.serve_horizon else wall
-    tph = len(done) * 3600.0 / horizon
-    out_line = {
+
+def serve_only(a):
+    rec = run_serve(a.serve_count, a.serve_rate, a.serve_n_lo, a.serve_n_hi, a.serve_horizon, a.serve_hbm_gb,
+                    a.serve_slots, a.serve_order, a.serve_checkpoint_dir, a.serve_checkpoint_secs, verbose=True)
+    line = {
         "metric": "serving: completed tests/hour (Poisson arrivals, 50% Fermat + 50% LLR, n log-uniform)",
-        "value": round(tph, 1),
+        "value": rec["completed_tests_per_hour"],
         "unit": "tests/hour",
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "f64",
         "data": "synthetic request stream (seeded)",
-        "config": {"workload": "serve_stream", "requests": a.serve_count, "rate_per_s": a.serve_rate,
-                   "n_range": [a.serve_n_lo, a.serve_n_hi], "horizon_s": a.serve_horizon,
-                   "hbm_budget_gib": a.serve_hbm_gb, "block": 1000, "slots": a.serve_slots,
-                   "order": a.serve_order},
-        "completed": len(done),
-        "completed_fermat": sum(1 for j, _ in done if j[0] == kite.JOB_PRP),
-        "completed_llr": sum(1 for j, _ in done if j[0] == kite.JOB_LLR),
-        "backlog": sum(1 for r in out if r.status == kite.JOB_PENDING),
-        "failed": sum(1 for r in out if r.status not in (0, kite.JOB_PENDING)),
-        "latency_p50_s": pct(0.50),
-        "latency_p99_s": pct(0.99),
-        "max_completed_n": max((j[2] for j, _ in done), default=0),
-        "wall_s": round(wall, 2),
-        "maxerr": round(max((r.maxerr for _, r in done), default=0.0), 4),
+        "config": {"workload": "serve_stream"},
+        "serve": rec,
     }
-    if a.serve_checkpoint_dir:  # restartable serving: the stopped jobs' states are on disk
-        out_line["checkpoint_dir"] = a.serve_checkpoint_dir
-        out_line["resumed"] = sum(1 for r in out if r.resumed)
-        out_line["checkpoint_files"] = sum(1 for f in os.listdir(a.serve_checkpoint_dir) if f.endswith(".ckpt"))
-    print(json.dumps(out_line), flush=True)
+    print(json.dumps(line), flush=True)
 
+
+# ---- the Gerbicz-checked production loop ---------------------------------------------------------
+
+def gerbicz_leg(tests, secs, blocks):
+    """The tests through kite_batch_run for `secs`: Fermat from x_0 = 3^k, Gerbicz blocks of 1000
+    squarings with their multiplies, a check every `blocks` blocks, round-off read every round. The
+    rate counts squarings up to each test's last passed check, from the moment every test had been
+    admitted (their starts, x_0 = 3^k by powering, come first)."""
+    from ahnum_amd import kite
+    jobs = [(kite.JOB_PRP, k, n, 0.0) for k, n in tests]
+    t0 = time.perf_counter()
+    rc, out = kite.batch_run(jobs, block=1000, blocks=blocks, horizon=secs)
+    wall = time.perf_counter() - t0
+    span = max(min(wall, secs) - max(r.start for r in out), 1e-9)
+    verified = sum(r.verified for r in out)
+    done = sum(r.iter for r in out)
+    tph = sum(r.verified / span * 3600.0 / n for r, (_, n) in zip(out, tests))
+    return {
+        "tests_per_hour": round(tph, 1),
+        "gerbicz_block": 1000,
+        "blocks_per_check": blocks,
+        "seconds": round(span, 2),
+        "verified_squarings": verified,
+        "squarings": done,
+        "gerbicz_errors": sum(r.gerbicz_errors for r in out),
+        "roundoff_retries": sum(r.roundoff_retries for r in out),
+        "maxerr": round(max(r.maxerr for r in out), 4),
+        "failed": sum(1 for r in out if r.status not in (0, kite.JOB_PENDING)),
+    }
+
+
+# ---- main --------------------------------------------------------------------------------------
 
 def main():
     ap = argparse.ArgumentParser()
@@ -171,6 +273,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10000, help="squarings of the one-test microbench")
     ap.add_argument("--cpu-secs", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gerbicz-secs", type=float, default=30.0,
+                    help="length of the Gerbicz-checked leg (0: skip)")
+    ap.add_argument("--gerbicz-blocks", type=int, default=16,
+                    help="blocks of 1000 squarings between checks in the Gerbicz-checked leg")
+    ap.add_argument("--serve-secs", type=float, default=30.0, help="horizon of the serving leg (0: skip)")
     ap.add_argument("--time-every", type=int, default=10,
                     help="bracket the launches of every N-th step of the kernel-timing region with HIP events")
     ap.add_argument("--time-steps", type=int, default=1000, help="steps of the kernel-timing region")
@@ -179,7 +286,7 @@ def main():
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a fill, a copy and an add over this many MiB (calibrates the HBM "
                          "byte counters of a rocprofv3 --pmc run of this command)")
-    ap.add_argument("--serve", action="store_true", help="serving run instead of the batch benchmark")
+    ap.add_argument("--serve", action="store_true", help="serving run only (the options below)")
     ap.add_argument("--serve-count", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-n-lo", type=int, default=200000)
@@ -199,7 +306,7 @@ def main():
         sys.exit(2)
 
     if a.serve:
-        serve(a)
+        serve_only(a)
         return
 
     from ahnum_amd import kite, workloads
@@ -227,7 +334,7 @@ def main():
     with kite.Ctx(k0, n0) as c:
         L0 = c.L
         x = c.num(workloads.random_residue_limbs(c.N, 2))
-        x.square(a.warmup)
+        x.square(max(a.warmup, 64))
         c.maxerr()
         c.reset_maxerr()
         t0 = time.perf_counter()
@@ -247,10 +354,13 @@ def main():
         b.set(s, 0, workloads.random_residue_limbs(k * (1 << n) - 1, 100 + i))
         Ls.append(b.fft_len(s))
     # timed region 1 (the value): square calls of `block` steps, as the Gerbicz loop makes them,
-    # replayed from the captured HIP graph
+    # replayed from the captured HIP graph; before it, the capture and one full warm replay of the
+    # same call, then the warm-up steps
     block = min(a.block, a.steps)
-    b.square(block)  # captures the graph
-    b.square(max(a.warmup - block, 0))
+    b.square(block)  # captures the graph (and runs it)
+    b.square(block)  # one warm replay
+    for _ in range(a.warmup // block):
+        b.square(block)
     b.maxerr(reset=True)
     b.sync()
     t0 = time.perf_counter()
@@ -302,6 +412,8 @@ def main():
             "batch": len(tests),
             "fft_lengths": sorted(set(Ls)),
             "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams",
+            "value_is": "squaring throughput of the timed steps as tests/hour (sum of squarings/s * 3600 / n); "
+                        "the production loop with Gerbicz checks is timed apart in gerbicz_checked",
             "seed": seed,
         },
         "roofline": {
@@ -325,9 +437,6 @@ def main():
             "maxerr": round(plain_err, 4),
             "number": f"{k0}*2^{n0}-1",
         },
-        # the whole step against both rooflines, in the algorithmic units of a squaring: 32 L bytes
-        # (two passes, each reading and writing the 8L-byte residue) and 5 L log2(L/2) + 20 L flops
-        # (forward and inverse complex FFTs of L/2 points, split square and carry/weights)
         "path_roofline": {
             "bytes_per_step": int(step_bytes),
             "flops_per_step": int(step_flops),
@@ -341,6 +450,14 @@ def main():
         "maxerr": round(max(errs.values()), 4),
         "gpu_ms_per_timed_step": round(sum(ms) / max(launches[0], 1), 4),
     }
+    if a.gerbicz_secs > 0:
+        g = gerbicz_leg(tests, a.gerbicz_secs, a.gerbicz_blocks)
+        g["vs_value"] = round(g["tests_per_hour"] / out["value"], 4) if out["value"] else None
+        out["gerbicz_checked"] = g
+    if a.serve_secs > 0 and a.workload == "single_large":
+        out["serve"] = run_serve(count=int(a.serve_secs * a.serve_rate), rate=a.serve_rate, n_lo=a.serve_n_lo,
+                                 n_hi=a.serve_n_hi, horizon=a.serve_secs, hbm_gb=a.serve_hbm_gb, slots=64,
+                                 order_name="least_work")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tests[:16], a.cpu_secs)
     print(json.dumps(out), flush=True)

@@ -122,6 +122,10 @@ typedef struct {
     double start, finish;  // seconds after the start of kite_batch_run (finish: result known)
     uint64_t resumed_at;   // iteration restored from a checkpoint (0: none, or an LLR run's start)
     uint32_t resumed;      // 1 if the job started from a checkpoint
+    uint32_t run;          // LLR: the run in progress or last done (0, or a confirmation run)
+    uint64_t iter;         // squarings done in the current run (Fermat: of the chain x; LLR: doublings)
+    uint64_t verified;     // Fermat: iteration of the last state that passed a Gerbicz check;
+                           // LLR: step of the last block that passed its round-off check
 } kite_job_result;
 
 typedef struct {
@@ -145,8 +149,10 @@ typedef struct {
     const char *checkpoint_dir;
     double checkpoint_secs;
     // Admission order among the jobs that have arrived: KITE_ORDER_ARRIVAL (first come, first
-    // served) or KITE_ORDER_LEAST_WORK (least n^2 first: with a slot limit, short tests finish
-    // while long ones wait, which lowers the latency of most requests at the same throughput).
+    // served) or KITE_ORDER_LEAST_WORK (least work first, work = n times the FFT length, twice
+    // for LLR jobs, whose composite answers are confirmed by a second run: with a slot limit,
+    // short tests finish while long ones wait). A job that has waited more than 60 s goes first
+    // whatever its work, so a steady stream of short jobs cannot hold a long one back for ever.
     uint32_t order;
 } kite_run_opts;
 

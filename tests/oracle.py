@@ -4,7 +4,8 @@ import ctypes
 import os
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PATH = os.path.join(ROOT, "oracle", "bin", "liboracle.so")
+# KITE_ORACLE_LIB: another build of the oracle (the benchmark's CPU baseline times the -O3 build)
+PATH = os.environ.get("KITE_ORACLE_LIB") or os.path.join(ROOT, "oracle", "bin", "liboracle.so")
 
 P = ctypes.c_void_p
 u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64

@@ -208,12 +208,16 @@ def test_serving_arrivals_slots_and_horizon():
 
 
 def test_least_work_admission_order():
-    """KITE_ORDER_LEAST_WORK with one slot: jobs that arrive together start smallest n first, and
-    each result is the known one (tests/known_residues.csv)."""
+    """KITE_ORDER_LEAST_WORK with one slot: jobs that arrive together start in order of their work,
+    squarings times FFT length (n L), and each result is the known one (tests/known_residues.csv)."""
     jobs = [(kite.JOB_PRP, 5, 30000, 0.0), (kite.JOB_PRP, 8191, 25000, 0.0), (kite.JOB_PRP, 9357, 20000, 0.0)]
+    work = [n * kite.lib().kite_choose_fft_len(k, n) for _, k, n, _ in jobs]
+    assert len(set(work)) == 3
     rc, out = kite.batch_run(jobs, slots=1, block=1000, order=kite.ORDER_LEAST_WORK)
     assert rc == 0
-    assert out[2].start <= out[2].finish <= out[1].start <= out[1].finish <= out[0].start
+    order = sorted(range(3), key=lambda i: work[i])
+    for a, b in zip(order, order[1:]):
+        assert out[a].start <= out[a].finish <= out[b].start
     assert ["%016X" % r.res64 for r in out] == ["B7980452FDBC4183", "D45538E6F07266C6", "79291CD2F47CE48C"]
 
 
