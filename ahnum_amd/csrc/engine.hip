@@ -72,6 +72,27 @@ extern "C" __global__ void k_set_u64(unsigned long long *p, unsigned long long v
     if (i < n) p[i] = v;
 }
 
+// Digit equality of two residues of each slot (kite_batch_equal): the first residue's digits were
+// written to the slot's operand-spectrum buffer, the second's to its digit buffer. One work-group
+// per slot; eq[s] = 1 if every digit agrees, 0 if any differs, -1 for a slot not compared.
+__global__ void k_digits_equal(const kite::SlotDev *__restrict__ slots, int nslots, int *__restrict__ eq) {
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const kite::SlotDev &d = slots[s];
+    __shared__ int differ;
+    if (threadIdx.x == 0) differ = 0;
+    __syncthreads();
+    if (d.count > 0) {
+        const int32_t *a = reinterpret_cast<const int32_t *>(d.yfreq), *b = d.dig;
+        const int L = 2 * d.R * d.C;
+        int mine = 0;
+        for (int i = threadIdx.x; i < L; i += blockDim.x) mine |= a[i] != b[i];
+        if (mine) differ = 1;  // every writer stores the same value
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) eq[s] = d.count > 0 ? !differ : -1;
+}
+
 // ================================================================================================
 // Host side
 // ================================================================================================
@@ -769,6 +790,8 @@ struct kite_batch {
     hipEvent_t evfork = nullptr, evtable = nullptr, evjoin[kMaxParts] = {};
     std::vector<int> slot_part;
     unsigned long long *d_maxerr = nullptr;
+    unsigned long long *d_cvterr = nullptr;  // round-off of digit conversions (kept apart)
+    int *d_eq = nullptr, *h_eq = nullptr;     // kite_batch_equal's answers (h_eq pinned)
     bool dirty = true;
     uint64_t gen = 0;  // bumped whenever the job arrays or groups are rebuilt
     // square calls replayed from a captured graph: valid for gsteps steps of generation ggen
@@ -853,6 +876,14 @@ static int batch_refresh(kite_batch *b) {
         BATCH_OK(b, hipHostMalloc((void **)&b->h_slots, nc * sizeof(SlotDev), hipHostMallocDefault));
         BATCH_OK(b, hipMalloc(&b->d_maxerr, nc * 8));
         BATCH_OK(b, hipMemsetAsync(b->d_maxerr, 0, nc * 8, b->stream));
+        if (b->d_cvterr) hipFree(b->d_cvterr);
+        if (b->d_eq) hipFree(b->d_eq);
+        if (b->h_eq) hipHostFree(b->h_eq);
+        b->d_cvterr = nullptr;
+        b->d_eq = b->h_eq = nullptr;
+        BATCH_OK(b, hipMalloc(&b->d_cvterr, nc * 8));
+        BATCH_OK(b, hipMalloc(&b->d_eq, nc * sizeof(int)));
+        BATCH_OK(b, hipHostMalloc((void **)&b->h_eq, nc * sizeof(int), hipHostMallocDefault));
         if (old) {
             BATCH_OK(b, hipMemcpyAsync(b->d_maxerr, old, b->slot_cap * 8, hipMemcpyDeviceToDevice, b->stream));
             BATCH_OK(b, hipStreamSynchronize(b->stream));
@@ -1015,7 +1046,10 @@ static int batch_refresh(kite_batch *b) {
 }
 
 // Fills the slot table: residue `x` (and operand `y`) per slot, step count per slot (0 = idle).
-static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, const uint32_t *count) {
+// digits_aside: a digits-mode pass writes into the slot's operand-spectrum buffer instead of its
+// digit buffer, and its rounding error into the conversion accumulator.
+static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, const uint32_t *count,
+                        bool digits_aside = false) {
     if (batch_refresh(b)) return -1;
     if (!batch_sync_all(b)) return -1;  // the previous table may still be read
     for (size_t s = 0; s < b->slots.size(); s++) {
@@ -1033,6 +1067,10 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
         d.xadd = yi ? 0 : b->slots[s].xadd;  // squarings only
         d.x = on ? (double2 *)b->slots[s].nums[xi[s]]->dev : nullptr;
         d.y_src = (on && yi && yi[s] >= 0) ? (const double2 *)b->slots[s].nums[yi[s]]->dev : nullptr;
+        if (digits_aside) {
+            d.dig = reinterpret_cast<int32_t *>(d.yfreq);  // 8 L bytes: room for L int32 digits
+            d.maxerr = b->d_cvterr + s;
+        }
     }
     BATCH_OK(b, hipMemcpyAsync(b->d_slots, b->h_slots, b->slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice,
                                b->stream));
@@ -1106,6 +1144,9 @@ extern "C" void kite_batch_free(kite_batch *b) {
     if (b->d_rows) hipFree(b->d_rows);
     if (b->d_cols) hipFree(b->d_cols);
     if (b->d_maxerr) hipFree(b->d_maxerr);
+    if (b->d_cvterr) hipFree(b->d_cvterr);
+    if (b->d_eq) hipFree(b->d_eq);
+    if (b->h_eq) hipHostFree(b->h_eq);
     for (int p = 0; p < kite_batch::kMaxParts; p++) {
         if (b->ps[p]) hipStreamDestroy(b->ps[p]);
         if (b->evjoin[p]) hipEventDestroy(b->evjoin[p]);
@@ -1311,6 +1352,32 @@ extern "C" int kite_batch_copy(kite_batch *b, const int32_t *dst, const int32_t 
         BATCH_OK(b, hipMemcpyAsync(bs.nums[dst[s]]->dev, bs.nums[src[s]]->dev, (size_t)bs.ctx->eng->H * sizeof(double2),
                                    hipMemcpyDeviceToDevice, b->ps[b->slot_part[s]]));
     }
+    return 0;
+}
+
+extern "C" int kite_batch_equal(kite_batch *b, const int32_t *a, const int32_t *c, int32_t *eq) {
+    if (b->err != hipSuccess) return -1;
+    const size_t cap = b->slots.size();
+    std::vector<uint32_t> one(cap, 0);
+    bool any = false;
+    for (size_t s = 0; s < cap; s++) {
+        one[s] = b->slots[s].ctx && a[s] >= 0 && c[s] >= 0;
+        any |= one[s] != 0;
+        eq[s] = -1;
+    }
+    if (!any) return 0;
+    // the digits of a[s] into the operand-spectrum buffer, then those of c[s] into the digit buffer
+    // (digit mode of the inverse column pass: the stored form's digits, each rounded to an integer)
+    if (batch_upload(b, a, nullptr, one.data(), true)) return -1;
+    BATCH_OK(b, batch_launch(b, 1, kModeDigits, 0));
+    if (batch_upload(b, c, nullptr, one.data())) return -1;
+    BATCH_OK(b, batch_launch(b, 1, kModeDigits, 0));
+    if (!batch_sync_all(b)) return -1;
+    hipLaunchKernelGGL(k_digits_equal, dim3((unsigned)cap), dim3(256), 0, b->stream, b->d_slots, (int)cap, b->d_eq);
+    BATCH_OK(b, hipGetLastError());
+    BATCH_OK(b, hipMemcpyAsync(b->h_eq, b->d_eq, cap * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BATCH_OK(b, hipStreamSynchronize(b->stream));
+    for (size_t s = 0; s < cap; s++) eq[s] = b->h_eq[s];
     return 0;
 }
 

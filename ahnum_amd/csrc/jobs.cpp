@@ -57,7 +57,9 @@ uint32_t default_block(uint64_t n) {
 uint32_t default_blocks(uint32_t B) { return std::max<uint32_t>(B / 4, 8); }
 
 enum Phase { kBlocks, kCheck, kTailX, kTailT, kLlrFirst, kLlrSteps, kDone };
-enum FermatNum { kX = 0, kD = 1, kX0 = 2, kT = 3 };
+// Fermat: x, the product d, x_0, t (d before its last update, or the second tail), and the last
+// verified (x, d), kept on the device between checks
+enum FermatNum { kX = 0, kD = 1, kX0 = 2, kT = 3, kVX = 4, kVD = 5 };
 enum LlrNum { kV = 1 };  // LLR: x (kX) and its last verified copy
 
 // What a caller asks of one test.
@@ -107,7 +109,8 @@ struct Job {
     // Fermat: i = squarings done; blocks of B count from x0_at (0, or a restart point)
     uint64_t i = 0, full = 0, check_at = 0, blk = 0;
     uint64_t vi = 0, ui = 0;                 // verified and unverified (from file) positions
-    std::vector<uint64_t> vx, vd, ux, ud;    // their x and d
+    std::vector<uint64_t> vx, vd, ux, ud;    // their x and d (vx, vd: host copies of kVX, kVD)
+    bool host_stale = false;                 // vx, vd are older than kVX, kVD
     std::vector<uint64_t> x0;                // x_0, when it is not base^k (a restart point)
     uint64_t x0_at = 0;
     uint32_t fails = 0, tail_tries = 0;
@@ -121,28 +124,36 @@ struct Job {
     int lzero[3] = {0, 0, 0};
     bool copy_pending = false;  // x passed its block: copy it to the verified number next round
     bool roundoff_pending = false, ladder_roundoff_pending = false;
+    double credit = 0;  // KITE_SHARE_TIME: rounds owed to this job (it takes part at >= 1)
+    std::vector<std::pair<int, int>> copies;  // device copies (dst, src) due before the next round
 };
 
-void power_start(kite_ctx *ctx, kite_num *x0, uint64_t k, uint32_t base) {
-    // base^k mod N: left-to-right binary powering, replaced by the exact power while it is below N
+// x_0 = base^k mod N into x0 and its limbs into l: the exact power, computed on the host, while it
+// is below N (always, for k below 2^30 and the n this engine runs); else left-to-right binary
+// powering on the device.
+void power_start(kite_ctx *ctx, kite_num *x0, uint64_t k, uint32_t base, std::vector<uint64_t> &l) {
+    l.assign(ctx->nlimbs, 0);
+    if ((double)k * std::log2((double)base) + 2 < (double)ctx->n) {
+        l[0] = 1;
+        size_t used = 1;
+        for (uint64_t e = 0; e < k; e++) {
+            unsigned __int128 c = 0;
+            for (size_t j = 0; j < used; j++) {
+                c += (unsigned __int128)l[j] * base;
+                l[j] = (uint64_t)c;
+                c >>= 64;
+            }
+            if (c) l[used++] = (uint64_t)c;
+        }
+        kite_set_limbs(ctx, x0, l.data());
+        return;
+    }
     kite_set_u64(ctx, x0, 1);
     for (int b = 63 - __builtin_clzll(k); b >= 0; b--) {
         kite_square(ctx, x0);
         if ((k >> b) & 1) kite_mul_small(ctx, x0, base);
     }
-    if ((double)k * std::log2((double)base) + 2 < (double)ctx->n) {
-        std::vector<uint64_t> l(ctx->nlimbs, 0);
-        l[0] = 1;
-        for (uint64_t e = 0; e < k; e++) {
-            unsigned __int128 c = 0;
-            for (uint64_t &w : l) {
-                c += (unsigned __int128)w * base;
-                w = (uint64_t)c;
-                c >>= 64;
-            }
-        }
-        kite_set_limbs(ctx, x0, l.data());
-    }
+    kite_get_limbs(ctx, x0, l.data());
 }
 
 std::string ckpt_name(const Spec &s) {
@@ -164,7 +175,7 @@ struct Scheduler {
     kite_batch *b = nullptr;
     size_t max_slots = (size_t)-1;
     uint64_t hbm_budget = 0;
-    uint32_t order = KITE_ORDER_ARRIVAL;
+    uint32_t order = KITE_ORDER_ARRIVAL, share = KITE_SHARE_SQUARINGS;
     double maxround = 0.40, horizon = 0, t0 = 0;
     void (*on_done)(void *, size_t, const kite_job_result *) = nullptr;
     void *done_arg = nullptr;
@@ -202,20 +213,34 @@ struct Scheduler {
     // (Re)starts Fermat job t in a fresh slot at length L from its verified state (or from the
     // unverified one, if newer).
     int start_fermat(Job &t, uint32_t L) {
-        if (add_slot(t, L, 4)) return -1;
+        if (add_slot(t, L, 6)) return -1;
         kite_ctx *c = ctx(t);
         kite_num *x0 = num(t, kX0);
-        if (t.x0.empty()) power_start(c, x0, t.sp.k, t.sp.base);
-        else kite_set_limbs(c, x0, t.x0.data());
+        std::vector<uint64_t> l0;
+        if (t.x0.empty()) {
+            power_start(c, x0, t.sp.k, t.sp.base, l0);
+        } else {
+            l0 = t.x0;
+            kite_set_limbs(c, x0, l0.data());
+        }
         if (t.vx.empty()) {  // a fresh start: x = d = x_0
-            t.vx.assign(c->nlimbs, 0);
-            kite_get_limbs(c, x0, t.vx.data());
-            t.vd = t.vx;
+            t.vx = l0;
+            t.vd = l0;
             t.vi = 0;
         }
         const bool from_u = t.ui > t.vi;
-        kite_set_limbs(c, num(t, kX), from_u ? t.ux.data() : t.vx.data());
-        kite_set_limbs(c, num(t, kD), from_u ? t.ud.data() : t.vd.data());
+        // every residue equal to x_0 is a device copy of it; the others come from their limbs
+        auto load = [&](int i, const std::vector<uint64_t> &v) {
+            if (v == l0) kite_copy(c, num(t, i), x0);
+            else kite_set_limbs(c, num(t, i), v.data());
+        };
+        load(kVX, t.vx);
+        load(kVD, t.vd);
+        load(kX, from_u ? t.ux : t.vx);
+        load(kD, from_u ? t.ud : t.vd);
+        engine_sync(c);  // the copies ran on the context's stream; the batch's launches come next
+        t.host_stale = false;
+        t.copies.clear();
         t.i = from_u ? t.ui : t.vi;
         t.pi = from_u ? t.up : t.vp;
         t.blk = 0;
@@ -223,17 +248,42 @@ struct Scheduler {
         t.full = t.x0_at + (t.sp.n - t.x0_at) / t.sp.B * t.sp.B;
         t.check_at = std::min(t.i + (uint64_t)t.sp.B * t.sp.M, t.full);
         t.phase = t.i >= t.full ? kTailX : kBlocks;
-        if (t.phase == kTailX) begin_tail(t);
+        if (t.phase == kTailX) {
+            kite_set_limbs(c, num(t, kX), t.vx.data());
+            kite_set_limbs(c, num(t, kT), t.vx.data());
+            begin_tail(t);
+        }
         return engine_ok(t) ? 0 : -1;
     }
 
+    // The tail from the verified state at `full` (x and t hold it, or get it by the copies due).
     void begin_tail(Job &t) {
-        kite_set_limbs(ctx(t), num(t, kX), t.vx.data());
-        kite_set_limbs(ctx(t), num(t, kT), t.vx.data());
         t.i = t.full;
         t.pi = t.vp;
         t.pending.clear();
         t.phase = kTailX;
+    }
+
+    // x and d back to the verified state, by device copies before the next round
+    void roll_back(Job &t) {
+        t.copies.push_back({kX, kVX});
+        t.copies.push_back({kD, kVD});
+        t.i = t.vi;
+        t.pi = t.vp;
+        t.blk = 0;
+        t.pending.clear();
+        t.check_at = std::min(t.vi + (uint64_t)t.sp.B * t.sp.M, t.full);
+        t.phase = kBlocks;
+    }
+
+    // host copies of the verified state from residues ix, id (after the batch is synchronised)
+    void pull(Job &t, int ix, int id) {
+        const size_t nl = ctx(t)->nlimbs;
+        t.vx.resize(nl);
+        t.vd.resize(nl);
+        kite_get_limbs(ctx(t), num(t, ix), t.vx.data());
+        kite_get_limbs(ctx(t), num(t, id), t.vd.data());
+        t.host_stale = false;
     }
 
     // Reports the points captured since the last verified state; false if the caller refused one.
@@ -269,17 +319,19 @@ struct Scheduler {
     // d == x. The product continues only where the reference's reader can continue it too: from
     // base^k (no restart since) with the reference's own block for this n. Otherwise the file is
     // written with d = x, and the product here restarts at x as a reader's would.
+    // (With host copies of the verified state that are current.)
     void save_fermat(Job &t) {
         if (t.sp.ckpt.empty() || t.slot < 0) return;
         const bool continues = t.x0_at == 0 && (t.sp.product_any_block || t.sp.B == default_block(t.sp.n));
-        if (!continues && t.vi > 0 && t.vd != t.vx) {
+        const bool restart = !continues && t.vi > 0 && t.vd != t.vx;
+        if (restart && !t.sp.uckpt.empty()) {
+            // an unverified file written after this one chains from what this one says, so the
+            // product in flight restarts at the verified x as well (blocks stay aligned)
             t.x0 = t.vx;
             t.vd = t.vx;
-            kite_set_limbs(ctx(t), num(t, kX0), t.vx.data());
+            for (int i : {kX0, kX, kD, kVX, kVD}) kite_set_limbs(ctx(t), num(t, i), t.vx.data());
+            t.copies.clear();
             if (t.phase == kBlocks || t.phase == kCheck) {
-                // the product in flight restarts at the verified x too (blocks stay aligned)
-                kite_set_limbs(ctx(t), num(t, kX), t.vx.data());
-                kite_set_limbs(ctx(t), num(t, kD), t.vx.data());
                 t.i = t.vi;
                 t.pi = t.vp;
                 t.blk = 0;
@@ -287,10 +339,10 @@ struct Scheduler {
                 t.check_at = std::min(t.vi + (uint64_t)t.sp.B * t.sp.M, t.full);
                 t.phase = t.vi >= t.full ? kTailX : kBlocks;
             }
-            if (!t.sp.uckpt.empty()) remove(t.sp.uckpt.c_str());
+            remove(t.sp.uckpt.c_str());
         }
-        prp_checkpoint_write(t.sp.ckpt.c_str(), t.sp.k, t.sp.n, t.sp.base, t.vi, t.L, t.vx.data(), t.vd.data(),
-                             t.vx.size());
+        prp_checkpoint_write(t.sp.ckpt.c_str(), t.sp.k, t.sp.n, t.sp.base, t.vi, t.L, t.vx.data(),
+                             restart ? t.vx.data() : t.vd.data(), t.vx.size());
         t.last_ck = now_s();
     }
 
@@ -446,6 +498,7 @@ struct Scheduler {
             save_llr(t);
         }
         if (t.sp.kind == KITE_JOB_PRP) {
+            if (t.host_stale) pull(t, kVX, kVD);  // the verified state moves to the new length by limbs
             t.ui = 0;  // the unverified state is abandoned with the failed attempt
             if (!t.sp.uckpt.empty()) remove(t.sp.uckpt.c_str());
         }
@@ -558,6 +611,7 @@ struct Scheduler {
             if (t.sp.kind == KITE_JOB_LLR && t.run > 0 && t.sp.on_confirm) t.sp.on_confirm(t.sp.progress_arg, t.run);
             t.last_ck = t.last_u = now_s();
             const int r = start(t, t.L);
+            t.res.start = now_s() - t0;  // in the rounds from now on
             if (r) {
                 done(t, r);
                 continue;
@@ -661,7 +715,18 @@ struct Scheduler {
     bool aborting(const Job &t) const { return t.sp.abort_flag && *t.sp.abort_flag; }
 
     // The Fermat state update after a round; false when the job has ended.
-    bool step_fermat(Job &t, uint32_t cnt, double e) {
+    // Residues a and c of job t hold the same value: the device's digit comparison (eq = 1), or,
+    // when the digits differ (eq = 0: equal values can still differ in a digit at a chunk's end),
+    // the exact comparison of their limbs.
+    bool same_value(Job &t, int a, int c, int eq) {
+        if (eq == 1) return true;
+        std::vector<uint64_t> la(ctx(t)->nlimbs), lc(ctx(t)->nlimbs);
+        kite_get_limbs(ctx(t), num(t, a), la.data());
+        kite_get_limbs(ctx(t), num(t, c), lc.data());
+        return la == lc;
+    }
+
+    bool step_fermat(Job &t, uint32_t cnt, double e, int eq) {
         switch (t.phase) {
         case kBlocks: {
             t.i += cnt;
@@ -688,14 +753,7 @@ struct Scheduler {
             return true;
         }
         case kCheck: {
-            std::vector<uint64_t> lt(ctx(t)->nlimbs), ld(ctx(t)->nlimbs);
-            kite_get_limbs(ctx(t), num(t, kT), lt.data());
-            kite_get_limbs(ctx(t), num(t, kD), ld.data());
-            if (!engine_ok(t)) {
-                done(t, -1);
-                return false;
-            }
-            if (lt != ld) {  // Gerbicz failure: back to the verified state
+            if (!same_value(t, kT, kD, eq)) {  // Gerbicz failure: back to the verified state
                 t.res.gerbicz_errors++;
                 if (!t.sp.uckpt.empty()) remove(t.sp.uckpt.c_str());
                 if (++t.fails > 5) {
@@ -703,45 +761,50 @@ struct Scheduler {
                     return false;
                 }
                 t.ui = 0;
-                kite_set_limbs(ctx(t), num(t, kX), t.vx.data());
-                kite_set_limbs(ctx(t), num(t, kD), t.vd.data());
-                t.i = t.vi;
-                t.pi = t.vp;
-                t.blk = 0;
-                t.pending.clear();
-                t.check_at = std::min(t.vi + (uint64_t)t.sp.B * t.sp.M, t.full);
-                t.phase = kBlocks;
+                roll_back(t);
                 if (t.stopping) {
+                    if (t.host_stale) pull(t, kVX, kVD);
                     save_fermat(t);
                     done(t, KITE_PRP_ABORTED);
                     return false;
                 }
                 return true;
             }
+            if (!engine_ok(t)) {
+                done(t, -1);
+                return false;
+            }
+            // verified: (x, d) become the state a later failure rolls back to
             t.fails = 0;
             t.ui = 0;
-            t.vx.resize(ctx(t)->nlimbs);
-            kite_get_limbs(ctx(t), num(t, kX), t.vx.data());
-            t.vd = ld;
             t.vi = t.i;
             t.vp = t.pi;
+            t.copies.push_back({kVX, kX});
+            t.copies.push_back({kVD, kD});
+            t.host_stale = true;
             if (!t.pending.empty() && !flush_points(t)) {
                 done(t, KITE_PRP_POINT_FAILED);
                 return false;
             }
             t.phase = kBlocks;
             t.check_at = std::min(t.vi + (uint64_t)t.sp.B * t.sp.M, t.full);
-            if (now_s() - t.last_ck >= t.sp.ck_secs || t.stopping || (t.sp.save_at_end && t.vi >= t.full))
-                save_fermat(t);
+            const bool at_full = t.vi >= t.full;
+            if (now_s() - t.last_ck >= t.sp.ck_secs || t.stopping || (t.sp.save_at_end && at_full) ||
+                (at_full && t.full == t.sp.n)) {
+                pull(t, kX, kD);  // x and d are the verified state now
+                if (now_s() - t.last_ck >= t.sp.ck_secs || t.stopping || (t.sp.save_at_end && at_full))
+                    save_fermat(t);
+            }
             if (t.stopping) {
                 done(t, KITE_PRP_ABORTED);
                 return false;
             }
-            if (t.vi >= t.full) {
+            if (at_full) {
                 if (t.full == t.sp.n) {
                     finish_fermat(t, t.vx, 0);
                     return false;
                 }
+                t.copies.push_back({kT, kX});  // the second tail starts from the same x
                 begin_tail(t);
             }
             return true;
@@ -751,26 +814,30 @@ struct Scheduler {
             if (t.i >= t.sp.n) t.phase = kTailT;
             return true;
         case kTailT: {
-            std::vector<uint64_t> lx(ctx(t)->nlimbs), lt(ctx(t)->nlimbs);
-            kite_get_limbs(ctx(t), num(t, kX), lx.data());
-            kite_get_limbs(ctx(t), num(t, kT), lt.data());
+            const bool same = same_value(t, kX, kT, eq);
             if (!engine_ok(t)) {
                 done(t, -1);
                 return false;
             }
-            if (lx == lt) {
+            if (same) {
                 if (!t.pending.empty() && !flush_points(t)) {
                     done(t, KITE_PRP_POINT_FAILED);
                     return false;
                 }
+                std::vector<uint64_t> lx(ctx(t)->nlimbs);
+                kite_get_limbs(ctx(t), num(t, kX), lx.data());
                 finish_fermat(t, lx, 0);
                 return false;
             }
             if (++t.tail_tries > 6) {
+                std::vector<uint64_t> lx(ctx(t)->nlimbs);
+                kite_get_limbs(ctx(t), num(t, kX), lx.data());
                 finish_fermat(t, lx, -3);
                 return false;
             }
             t.res.gerbicz_errors++;
+            t.copies.push_back({kX, kVX});  // both tails again from the verified x at `full`
+            t.copies.push_back({kT, kVX});
             begin_tail(t);
             return true;
         }
@@ -844,28 +911,64 @@ struct Scheduler {
             }
         }
         if (kite_batch_copy(b, vdst.data(), vsrc.data())) return false;
+        // which jobs take part: all of them, or (time share) those whose credit has reached a round
+        std::vector<char> due(cap, 1);
+        if (share == KITE_SHARE_TIME) {
+            uint32_t Lmin = ~0u;
+            for (size_t ti : active) Lmin = std::min(Lmin, jobs[ti].L);
+            for (size_t ti : active) {
+                Job &t = jobs[ti];
+                t.credit += (double)Lmin / (double)t.L;
+                due[t.slot] = t.credit >= 1.0 - 1e-9;
+                if (due[t.slot]) t.credit -= 1.0;
+            }
+        }
         for (size_t ti : active) {
             Job &t = jobs[ti];
-            cnt[t.slot] = plan(t, &which[t.slot], &cdst[t.slot], &csrc[t.slot], &mdst[t.slot], &msrc[t.slot]);
+            if (due[t.slot])
+                cnt[t.slot] = plan(t, &which[t.slot], &cdst[t.slot], &csrc[t.slot], &mdst[t.slot], &msrc[t.slot]);
         }
         if (kite_batch_square(b, which.data(), cnt.data())) return false;
         bool any_event = false;
         for (size_t ti : active) {
             const Job &t = jobs[ti];
+            if (!due[t.slot]) continue;
             const uint64_t at = t.i + cnt[t.slot];
             any_event |= (t.fault_pending && at == t.sp.fault_at) ||
                          (t.sp.on_point && t.pi < t.sp.npoints && t.sp.points[t.pi] == at);
         }
         if (any_event) {
             if (kite_batch_sync(b)) return false;
-            for (size_t ti : active) events(jobs[ti], cnt[jobs[ti].slot]);
+            for (size_t ti : active)
+                if (due[jobs[ti].slot]) events(jobs[ti], cnt[jobs[ti].slot]);
         }
         if (kite_batch_copy(b, cdst.data(), csrc.data()) || kite_batch_mul(b, mdst.data(), msrc.data())) return false;
         std::vector<double> err(cap, 0);
         if (kite_batch_maxerr(b, err.data(), 1)) return false;
+        // the checks of this round, compared on the device: t against d after a check's squarings,
+        // the two tails against each other
+        std::vector<int32_t> ea(cap, -1), ec(cap, -1), eq(cap, -1);
+        bool compare = false;
+        for (size_t ti : active) {
+            const Job &t = jobs[ti];
+            if (!due[t.slot] || err[t.slot] > maxround || t.sp.kind != KITE_JOB_PRP) continue;
+            if (t.phase == kCheck) {
+                ea[t.slot] = kT;
+                ec[t.slot] = kD;
+            } else if (t.phase == kTailT) {
+                ea[t.slot] = kX;
+                ec[t.slot] = kT;
+            }
+            compare |= ea[t.slot] >= 0;
+        }
+        if (compare && kite_batch_equal(b, ea.data(), ec.data(), eq.data())) return false;
         std::vector<size_t> still;
         for (size_t ti : active) {
             Job &t = jobs[ti];
+            if (!due[t.slot]) {  // sat this round out
+                still.push_back(ti);
+                continue;
+            }
             double e = err[t.slot];
             t.res.maxerr = std::max(t.res.maxerr, e);
             if (t.roundoff_pending && t.run == 0 && t.phase == kLlrSteps && t.i + cnt[t.slot] >= t.sp.roundoff_at) {
@@ -885,10 +988,32 @@ struct Scheduler {
                 still.push_back(ti);
                 continue;
             }
-            const bool going = t.sp.kind == KITE_JOB_PRP ? step_fermat(t, cnt[t.slot], e) : step_llr(t, cnt[t.slot], e);
+            const bool going = t.sp.kind == KITE_JOB_PRP ? step_fermat(t, cnt[t.slot], e, eq[t.slot])
+                                                         : step_llr(t, cnt[t.slot], e);
             if (going) still.push_back(ti);
         }
         active.swap(still);
+        return run_copies();
+    }
+
+    // The device copies the jobs' updates asked for, in order, one per job and call.
+    bool run_copies() {
+        const int cap = kite_batch_capacity(b);
+        for (size_t k = 0;; k++) {
+            std::vector<int32_t> dst(cap, -1), src(cap, -1);
+            bool any = false;
+            for (size_t ti : active) {
+                Job &t = jobs[ti];
+                if (k < t.copies.size()) {
+                    dst[t.slot] = t.copies[k].first;
+                    src[t.slot] = t.copies[k].second;
+                    any = true;
+                }
+            }
+            if (!any) break;
+            if (kite_batch_copy(b, dst.data(), src.data())) return false;
+        }
+        for (size_t ti : active) jobs[ti].copies.clear();
         return true;
     }
 
@@ -906,6 +1031,7 @@ struct Scheduler {
                 }
                 save_llr(t);
             } else if (t.vi > 0) {
+                if (t.host_stale) pull(t, kVX, kVD);
                 save_fermat(t);
             }
         }
@@ -942,6 +1068,7 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
     S.max_slots = o.slots ? o.slots : (size_t)-1;
     S.hbm_budget = o.hbm_budget;
     S.order = o.order;
+    S.share = o.share;
     S.maxround = o.max_roundoff > 0 ? o.max_roundoff : 0.40;
     S.horizon = o.horizon;
     S.on_done = o.on_done;

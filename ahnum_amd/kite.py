@@ -322,6 +322,7 @@ SIGNATURES.update({
     "kite_batch_square": (ctypes.c_int, [c_p, I32P, U32P]),
     "kite_batch_mul": (ctypes.c_int, [c_p, I32P, I32P]),
     "kite_batch_copy": (ctypes.c_int, [c_p, I32P, I32P]),
+    "kite_batch_equal": (ctypes.c_int, [c_p, I32P, I32P, I32P]),
     "kite_batch_maxerr": (ctypes.c_int, [c_p, ctypes.POINTER(c_dbl), ctypes.c_int]),
     "kite_batch_sync": (ctypes.c_int, [c_p]),
     "kite_batch_set_timing": (ctypes.c_int, [c_p, ctypes.c_int]),
@@ -336,6 +337,7 @@ SIGNATURES.update({
 
 JOB_PRP, JOB_LLR, JOB_PENDING = 0, 1, 1
 ORDER_ARRIVAL, ORDER_LEAST_WORK = 0, 1
+SHARE_SQUARINGS, SHARE_TIME = 0, 1  # kite_run_opts.share
 
 
 class Job(ctypes.Structure):
@@ -383,6 +385,7 @@ class RunOpts(ctypes.Structure):
         ("checkpoint_dir", ctypes.c_char_p),
         ("checkpoint_secs", c_dbl),
         ("order", c_u32),
+        ("share", c_u32),
     ]
 
 
@@ -398,7 +401,8 @@ SIGNATURES.update({
 
 
 def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fault_job=0, fault_at=0,
-              max_roundoff=0.0, on_done=None, checkpoint_dir=None, checkpoint_secs=0.0, order=ORDER_ARRIVAL):
+              max_roundoff=0.0, on_done=None, checkpoint_dir=None, checkpoint_secs=0.0, order=ORDER_ARRIVAL,
+              share=SHARE_SQUARINGS):
     """Runs jobs [(kind, k, n, arrival), ...] (kind JOB_PRP or JOB_LLR; Fermat base 3) through
     kite_batch_run, with checkpoint files in checkpoint_dir if given. Returns (rc, [JobResult])."""
     arr = (Job * len(jobs))(*[Job(k, n, kind, 3, 0, arrival) for kind, k, n, arrival in jobs])
@@ -412,6 +416,7 @@ def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fa
     o.inject_fault_job = fault_job
     o.inject_fault_at = fault_at
     o.order = order
+    o.share = share
     if checkpoint_dir is not None:
         o.checkpoint_dir = os.fsencode(checkpoint_dir)
         o.checkpoint_secs = checkpoint_secs

@@ -50,6 +50,11 @@ int kite_batch_set_xadd(kite_batch *b, int slot, int32_t xadd);
 int kite_batch_mul(kite_batch *b, const int32_t *dst, const int32_t *src);
 // For each slot s with dst[s] >= 0: residue dst[s] = residue src[s].
 int kite_batch_copy(kite_batch *b, const int32_t *dst, const int32_t *src);
+// For each slot s with a[s] >= 0 and c[s] >= 0: eq[s] = 1 if residues a[s] and c[s] have the same
+// digits (so the same value mod N), 0 if their digits differ (the values may still agree: digits
+// are balanced, but not canonical in every position; compare limbs to decide); -1 elsewhere. The
+// comparison runs on the device; only the answers come back.
+int kite_batch_equal(kite_batch *b, const int32_t *a, const int32_t *c, int32_t *eq);
 // out[s] = max round-off of slot s since its last reset (capacity entries); reset if asked.
 int kite_batch_maxerr(kite_batch *b, double *out, int reset);
 int kite_batch_sync(kite_batch *b);
@@ -101,6 +106,7 @@ int kite_batch_prp(const kite_batch_test *tests, size_t ntests, const kite_batch
 enum { KITE_JOB_PRP = 0, KITE_JOB_LLR = 1 };
 enum { KITE_JOB_PENDING = 1 };  // status of a job the horizon stopped before it finished
 enum { KITE_ORDER_ARRIVAL = 0, KITE_ORDER_LEAST_WORK = 1 };  // kite_run_opts.order
+enum { KITE_SHARE_SQUARINGS = 0, KITE_SHARE_TIME = 1 };      // kite_run_opts.share
 
 typedef struct {
     uint64_t k, n;
@@ -119,7 +125,8 @@ typedef struct {
     uint32_t fft_len;   // length of the final run
     double maxerr;
     uint32_t gerbicz_errors, roundoff_retries, confirm_runs;
-    double start, finish;  // seconds after the start of kite_batch_run (finish: result known)
+    double start, finish;  // seconds after the start of kite_batch_run: start, when the job's setup
+                           // (x_0 or the ladder, or its checkpoint) was done; finish, result known
     uint64_t resumed_at;   // iteration restored from a checkpoint (0: none, or an LLR run's start)
     uint32_t resumed;      // 1 if the job started from a checkpoint
     uint32_t run;          // LLR: the run in progress or last done (0, or a confirmation run)
@@ -154,6 +161,12 @@ typedef struct {
     // short tests finish while long ones wait). A job that has waited more than 60 s goes first
     // whatever its work, so a steady stream of short jobs cannot hold a long one back for ever.
     uint32_t order;
+    // How the jobs in flight share the rounds: KITE_SHARE_SQUARINGS (every job squares one block
+    // per round: the most work per launch, the highest throughput) or KITE_SHARE_TIME (a job of
+    // FFT length L takes part in a fraction L_min / L of the rounds, L_min the shortest length in
+    // flight: every job gets about the same device time per round, so a short test is not held to
+    // the pace of the longest ones in flight, which lowers serving latency).
+    uint32_t share;
 } kite_run_opts;
 
 // Device bytes a job of this kind holds at length L while in flight (the admission estimate).
