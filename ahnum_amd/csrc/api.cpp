@@ -78,6 +78,9 @@ extern "C" void kite_copy(kite_ctx *ctx, kite_num *dst, const kite_num *src) {
 
 extern "C" void kite_square(kite_ctx *ctx, kite_num *x) { engine_square(ctx, x->dev, 1); }
 extern "C" void kite_square_n(kite_ctx *ctx, kite_num *x, uint64_t count) { engine_square(ctx, x->dev, count); }
+extern "C" int kite_square_n_static(kite_ctx *ctx, kite_num *x, uint64_t count) {
+    return engine_square_static(ctx, x->dev, count);
+}
 
 extern "C" void kite_mul(kite_ctx *ctx, kite_num *x, const kite_num *y) { engine_mul(ctx, x->dev, y->dev); }
 extern "C" void kite_add(kite_ctx *ctx, kite_num *x, const kite_num *y) { engine_addsub(ctx, x->dev, y->dev, 1); }

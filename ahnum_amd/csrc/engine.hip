@@ -569,36 +569,40 @@ static bool set_slot(Engine *E, const SlotDev &s) {
     return chk(E, hipGetLastError(), "k_set_slot");
 }
 
-static bool launch_rows(Engine *E, int mode, int step = 0) {
+// The passes of this context read their slot from `tbl` (E->d_slot, or the static scratch of
+// kite_square_n_static).
+static bool launch_rows(Engine *E, int mode, int step = 0, const SlotDev *tbl = nullptr) {
+    if (!tbl) tbl = E->d_slot;
     if (mode == kModeSquare && E->krows_sq)
-        return chk(E, launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, E->d_slot, nullptr, mode, step,
-                                  kRowsSqThreads), "rows pass");
-    return chk(E, launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, E->d_slot, nullptr, mode, step),
-               "rows pass");
+        return chk(E, launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, tbl, nullptr, mode, step, kRowsSqThreads),
+                   "rows pass");
+    return chk(E, launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, tbl, nullptr, mode, step), "rows pass");
 }
 
-static bool launch_cols(Engine *E, int mode, int step = 0) {
-    return chk(E, launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, E->d_slot, nullptr, mode, step),
+static bool launch_cols(Engine *E, int mode, int step = 0, const SlotDev *tbl = nullptr) {
+    if (!tbl) tbl = E->d_slot;
+    return chk(E, launch_pass(E->kcinv, E->ncoljobs, E->lds_cols, E->stream, tbl, nullptr, mode, step),
                "column pass (inverse)") &&
-           chk(E, launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, E->d_slot, nullptr, mode, step),
+           chk(E, launch_pass(E->kcfwd, E->ncoljobs, E->lds_cols, E->stream, tbl, nullptr, mode, step),
                "column pass (forward)");
 }
 
 // One squaring (step `step` of the slot in E->d_slot) of this context's number alone: rows (one
 // pair per work-group where that kernel exists, since a single test leaves most CUs idle and
 // latency decides), then the fused column pass (step s uses seq + s + 1) or the two column passes.
-static bool launch_square(Engine *E, int step) {
+static bool launch_square(Engine *E, int step, const SlotDev *tbl = nullptr) {
+    if (!tbl) tbl = E->d_slot;
     bool ok;
     if (E->krows1)
-        ok = chk(E, launch_pass(E->krows1, E->nrowjobs1, E->lds_rows1, E->stream, E->d_slot, nullptr, kModeSquare,
-                                step), "rows pass");
+        ok = chk(E, launch_pass(E->krows1, E->nrowjobs1, E->lds_rows1, E->stream, tbl, nullptr, kModeSquare, step),
+                 "rows pass");
     else
-        ok = launch_rows(E, kModeSquare, step);
+        ok = launch_rows(E, kModeSquare, step, tbl);
     if (!ok) return false;
     if (E->kcfused)
-        return chk(E, launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, E->d_slot, nullptr, kModeSquare,
-                                  step), "fused column pass");
-    return launch_cols(E, kModeSquare, step);
+        return chk(E, launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, tbl, nullptr, kModeSquare, step),
+                   "fused column pass");
+    return launch_cols(E, kModeSquare, step, tbl);
 }
 
 // Squarings of one context in runs of kSqGraphSteps are replayed from a captured HIP graph whose
@@ -658,6 +662,83 @@ void engine_square(kite_ctx *ctx, void *x, uint64_t count, int32_t xadd) {
         E->seq += run;
         count -= run;
     }
+}
+
+// ---- squarings without allocation (kite_square_n_static) -------------------------------------
+//
+// Callers that may not allocate device memory (code under stream capture, tight serving loops,
+// harnesses that forbid it) square through this entry point: no graph, no allocation, the device
+// scratch of the squaring (the slot record its kernels read, and the carry flags of the fused
+// column pass) in this file-scope array. Every call on a GPU shares that GPU's copy, and each call
+// clears the flag words it uses, so calls that overlap on one GPU race; calls in sequence do not.
+constexpr size_t kScratchSlotBytes = 256;
+constexpr size_t kScratchBytes = (size_t)2 << 20;  // 2 MiB: the slot record and 262112 flag words
+constexpr size_t kScratchFlags = (kScratchBytes - kScratchSlotBytes) / 8;
+static_assert(sizeof(SlotDev) <= kScratchSlotBytes, "slot record fits its part of the scratch");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];  // the device address of `scratch`, per device
+
+// The address of this device's `scratch`, resolved once per device (the device must be current).
+static unsigned char *scratch_of_device(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            fprintf(stderr, "kite: %s\n", g_err.c_str());
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return static_cast<unsigned char *>(g_scratch_ptr[dev]);
+}
+
+int engine_square_static(kite_ctx *ctx, void *x, uint64_t count) {
+    Engine *E = ctx->eng;
+    if (E->err != hipSuccess) return -1;
+    int dev = -1;
+    if (!chk(E, hipGetDevice(&dev), "hipGetDevice")) return -1;
+    if (dev != E->shape->device) {
+        g_err = "kite_square_n_static: the context's device is not the current device";
+        fprintf(stderr, "kite: %s\n", g_err.c_str());
+        return -1;
+    }
+    unsigned char *ptr = scratch_of_device(dev);
+    if (!ptr) return -1;
+    SlotDev *tbl = reinterpret_cast<SlotDev *>(ptr);
+    unsigned long long *flags = reinterpret_cast<unsigned long long *>(ptr + kScratchSlotBytes);
+    const size_t nflags = (size_t)E->shape->host.NG * E->shape->host.R;
+    if (E->kcfused && nflags > kScratchFlags) {
+        g_err = "kite_square_n_static: this FFT length needs " + std::to_string(nflags) + " carry flags, the " +
+                "static scratch holds " + std::to_string(kScratchFlags);
+        fprintf(stderr, "kite: %s\n", g_err.c_str());
+        return -2;
+    }
+    if (E->kcfused) {  // the words this call waits on start below its first launch tag (1)
+        hipLaunchKernelGGL(k_set_u64, dim3((unsigned)((nflags + 255) / 256)), dim3(256), 0, E->stream, flags, 0ull,
+                           (int)nflags);
+        if (!chk(E, hipGetLastError(), "k_set_u64(scratch flags)")) return -1;
+    }
+    SlotDev s = E->slot;
+    s.x = (double2 *)x;
+    s.xadd = 0;
+    s.flags = flags;
+    for (uint64_t done = 0; done < count;) {
+        const uint32_t run = (uint32_t)std::min<uint64_t>(count - done, 1u << 20);
+        s.seq = (uint32_t)done;  // step i of this run waits for tag done + i + 1, as it was written
+        s.count = run;
+        hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, E->stream, tbl, s);
+        if (!chk(E, hipGetLastError(), "k_set_slot(scratch)")) return -1;
+        for (uint32_t i = 0; i < run; i++)
+            if (!launch_square(E, (int)i, tbl)) return -1;
+        done += run;
+    }
+    return chk(E, hipStreamSynchronize(E->stream), "hipStreamSynchronize(square_n_static)") ? 0 : -1;
 }
 
 void engine_mul(kite_ctx *ctx, void *x, const void *y) {

@@ -81,6 +81,8 @@ double engine_maxerr(kite_ctx *ctx);
 void engine_reset_maxerr(kite_ctx *ctx);
 void engine_sync(kite_ctx *ctx);
 int engine_error(const kite_ctx *ctx);  // 0, or -1 once a HIP call of this context has failed
+// count squarings of x with no allocation (static device scratch): 0, -1 (failure), -2 (too long)
+int engine_square_static(kite_ctx *ctx, void *x, uint64_t count);
 const char *engine_last_error();
 
 // ---- LLR arithmetic (llr_math.cpp) -----------------------------------------------------------

@@ -70,6 +70,17 @@ int kite_batch_pass_times(kite_batch *b, double *out3, uint64_t *launches3);
 int kite_batch_kernel_times(kite_batch *b, int i, char *name, size_t name_len, double *ms,
                             uint64_t *launches, double *bytes);
 
+// kite_square_n without allocation. No counterpart in the reference: it is for callers that cannot
+// allocate device memory (code under HIP stream capture, tight serving loops, harnesses that forbid
+// allocation). x is squared count times, like kite_square_n, but with direct launches (no graph)
+// and the squaring's device scratch (its slot record and the fused column pass's carry flags) in
+// a static 2 MiB array of the library, one per GPU, that holds carry flags for FFT lengths up to
+// about 2M. Returns 0 on success, -2 if the context's length needs more flags than the scratch
+// holds (nothing is done), -1 on a failure (the context's error is latched). Waits for the
+// squarings before it returns. Every call on one GPU shares that GPU's scratch: calls that overlap
+// on one GPU (from several threads) race; calls in sequence do not.
+int kite_square_n_static(kite_ctx *ctx, kite_num *x, uint64_t count);
+
 // Testing hook: the n-th HIP call the library checks from now on (in any context or batch) is
 // reported as failed (n <= 0: off). Nothing fails on the device; the library takes its error path:
 // the context's round-off reads +inf, batch calls return -1 and the drivers a negative code.
