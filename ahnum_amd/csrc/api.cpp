@@ -27,7 +27,12 @@ extern "C" kite_ctx *kite_ctx_new(uint64_t k, uint64_t n, int32_t c, const kite_
     ctx->nlimbs = limbs_for_n(n);
     ctx->Nl.assign(ctx->nlimbs, 0);
     ctx->tmpdig.assign(ctx->L, 0);
-    if (build_layout(ctx)) {
+    if (build_layout(ctx) || peak_product_bits(ctx) > kMaxProductBits) {
+        delete ctx;
+        return nullptr;
+    }
+    ctx->min_sub = carry_sub_needed(ctx);
+    if (!ctx->min_sub) {
         delete ctx;
         return nullptr;
     }

@@ -149,7 +149,7 @@ struct ShapeChoice {
     bool fast = false;
 };
 
-static bool choose_shape(uint32_t L, uint32_t top_digits, uint32_t variant, ShapeChoice *out) {
+static bool choose_shape(uint32_t L, uint32_t top_digits, uint32_t variant, uint32_t min_sub, ShapeChoice *out) {
     if (L % 2 || L < 64) return false;
     const uint32_t H = L / 2;
     std::vector<ShapeChoice> cands;
@@ -162,7 +162,7 @@ static bool choose_shape(uint32_t L, uint32_t top_digits, uint32_t variant, Shap
         const bool gfull = C % G == 0;  // the compile-time kernels use exactly g_for_R(R)
         while (C % G) G /= 2;           // the generic kernels take any power of two dividing C
         const uint32_t sub = G < (uint32_t)kSub ? G : (uint32_t)kSub;
-        if (top_digits && 2 * sub < top_digits) continue;
+        if ((top_digits && 2 * sub < top_digits) || sub < min_sub) continue;
         if ((2ull * R * G + R) * 16 + (uint64_t)R * (G / sub) * 8 > (uint64_t)kMaxLds) continue;
         const bool fast = gfull && fast_rows((int)C) && fast_cols_inv((int)R);
         double cost = fabs(log2((double)C) - log2((double)R) - 2.0) + (fast ? 0 : 4);
@@ -179,14 +179,14 @@ static bool choose_shape(uint32_t L, uint32_t top_digits, uint32_t variant, Shap
     return true;
 }
 
-bool engine_shape_ok(uint32_t L, uint32_t top_digits) {
+bool engine_shape_ok(uint32_t L, uint32_t top_digits, uint32_t min_sub) {
     ShapeChoice sc;
-    return choose_shape(L, top_digits, 0, &sc);
+    return choose_shape(L, top_digits, 0, min_sub, &sc);
 }
 
 bool engine_shape_fast(uint32_t L) {
     ShapeChoice sc;
-    return choose_shape(L, 0, 0, &sc) && sc.fast;
+    return choose_shape(L, 0, 0, 1, &sc) && sc.fast;
 }
 
 // Shapes (twiddle tables) are shared by every context of the same (R, C, G) on a device.
@@ -411,7 +411,7 @@ int engine_init(kite_ctx *ctx) {
     }
     set_lds_limits();
     ShapeChoice ch;
-    if (!choose_shape(ctx->L, ctx->L - ctx->top_t, ctx->plan_order, &ch)) {
+    if (!choose_shape(ctx->L, ctx->L - ctx->top_t, ctx->plan_order, ctx->min_sub, &ch)) {
         g_err = "no shape for this length";
         return -1;
     }

@@ -39,6 +39,7 @@ struct kite_ctx {
     uint64_t top_st = 0;       // s_t
     uint32_t top_m = 0;        // n - s_t
     uint32_t b0 = 0;           // floor(T'/L): every digit has b0 or b0+1 bits
+    uint32_t min_sub = 1;      // carry sub-chunk the engine's shape must have at least (elements)
     size_t nlimbs = 0;
     std::vector<uint64_t> Nl;  // N as limbs
     uint32_t err_every = 1;
@@ -51,6 +52,15 @@ namespace kite {
 // ---- layout (layout.cpp) --------------------------------------------------------------------
 uint64_t frac_log2_64(uint64_t k);
 int build_layout(kite_ctx *ctx);
+// Expected peak magnitude, in bits, of a product before its carry: log2 k + 2 (b0 + 1) + log2(L)/2.
+double peak_product_bits(const kite_ctx *ctx);
+// A length whose products would pass this many bits is refused: past 2^53 an FP64 product has no
+// fraction bits left, so its round-off can no longer be measured, and the digits would outgrow int32.
+constexpr double kMaxProductBits = 53.0;
+// The smallest carry sub-chunk (elements of 2 digits, a power of two) whose carry the next sub-chunk
+// absorbs with a leftover below 2^(b0 - 1); 0 if none up to 64 elements. Above 1 only for short digits
+// with a large k.
+uint32_t carry_sub_needed(const kite_ctx *ctx);
 // Exact fraction numerator of j*T'/L: (j * T'fix) mod (L*2^64), as a 128-bit value.
 unsigned __int128 phase_num(const kite_ctx *ctx, uint64_t j);
 unsigned __int128 tfix(const kite_ctx *ctx);  // T' * 2^64
@@ -67,7 +77,9 @@ bool fft_len_ok(uint32_t L);
 // ---- engine (engine.hip) ---------------------------------------------------------------------
 int engine_init(kite_ctx *ctx);
 void engine_free(kite_ctx *ctx);
-bool engine_shape_ok(uint32_t L, uint32_t top_digits);
+// A shape for L whose top region (top_digits, 0 if unknown) fits the last carry sub-chunk and whose
+// carry sub-chunks hold at least min_sub elements (2 digits each).
+bool engine_shape_ok(uint32_t L, uint32_t top_digits, uint32_t min_sub = 1);
 bool engine_shape_fast(uint32_t L);  // a shape with compile-time kernels exists
 void *engine_num_alloc(kite_ctx *ctx);
 void engine_num_free(kite_ctx *ctx, void *p);

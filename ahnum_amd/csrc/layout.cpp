@@ -73,6 +73,39 @@ int build_layout(kite_ctx *ctx) {  // src/ctx.c:198-221
     return 0;
 }
 
+// The inverse transform returns k times a length-L cyclic convolution of balanced digits of at
+// most b0 + 1 bits: about k 2^(2 b0 + 2) sqrt(L) at its largest (1.4 bits above the largest value
+// measured at k = 8045, n = 659321, L = 49152, where the round-off is 0.19). The lengths the selector
+// picks stay below 51 bits; a forced length above kMaxProductBits is refused.
+static double peak_bits(double log2k, uint32_t b0, uint32_t L) {
+    return log2k + 2.0 * (b0 + 1) + 0.5 * std::log2((double)L);
+}
+
+double peak_product_bits(const kite_ctx *ctx) { return peak_bits(std::log2((double)ctx->k), ctx->b0, ctx->L); }
+
+// The carry out of a sub-chunk of 2 sub digits (about peak - b0 bits) is absorbed by the next
+// sub-chunk's first 2 sub - 1 digits, and its last digit keeps what is left. That leftover has to stay
+// below 2^(b0 - 1): a larger one makes the next products, and so the next leftover, larger still
+// (k = 2^29 - 1 at 2.44 bits per digit and k = 2^25 + 1 at 2.04 diverge on the second squaring of
+// 8-element sub-chunks, with round-off 0.5). Short digits with a large k need the wider sub-chunks.
+static uint32_t needed_sub(double log2k, double bits, uint32_t b0, uint32_t L) {
+    for (uint32_t sub = 1; sub <= 64; sub *= 2)
+        if (peak_bits(log2k, b0, L) - b0 - (2.0 * sub - 1) * bits <= b0 - 1.0) return sub;
+    return 0;
+}
+
+uint32_t carry_sub_needed(const kite_ctx *ctx) {
+    const double log2k = std::log2((double)ctx->k);
+    return needed_sub(log2k, ((double)ctx->n + log2k) / ctx->L, ctx->b0, ctx->L);
+}
+
+// a shape for L that absorbs the carries of k*2^n-1, before any layout exists
+static bool shape_for(uint64_t k, uint64_t n, uint32_t L) {
+    const double log2k = std::log2((double)k), bits = ((double)n + log2k) / L;
+    const uint32_t sub = needed_sub(log2k, bits, (uint32_t)bits, L);
+    return sub && engine_shape_ok(L, 0, sub);
+}
+
 bool fft_len_ok(uint32_t L) {  // src/ctx.c:17-24
     if (L < 16 || (L & 1)) return false;
     for (uint32_t p : {2u, 3u, 5u, 7u})
@@ -208,7 +241,9 @@ double cost_model(uint32_t L) {  // odd radices cost more per point (src/ctx.c:9
     return (double)L * c;
 }
 
-bool usable(uint32_t L) { return L % 64 == 0 && fft_len_ok(L) && engine_shape_ok(L, 0); }
+bool usable(uint32_t L, uint64_t k, uint64_t n) {
+    return L % 64 == 0 && fft_len_ok(L) && shape_for(k, n, L);
+}
 }  // namespace
 
 uint32_t kite_tuned_order(uint32_t L) {
@@ -221,12 +256,12 @@ uint32_t kite_tuned_order(uint32_t L) {
 extern "C" uint32_t kite_choose_fft_len(uint64_t k, uint64_t n) {
     if (n < 2048) {  // tiny numbers: short power-of-two lengths
         for (uint32_t L = 64; L <= n; L *= 2)
-            if ((double)n / L <= kite_safe_bpw(L, k) && engine_shape_ok(L, 0)) return L;
+            if ((double)n / L <= kite_safe_bpw(L, k) && shape_for(k, n, L)) return L;
         return 0;
     }
     uint32_t lo = 0;
     for (uint32_t L = 64; L < (1u << 28); L += 64)
-        if (usable(L) && (double)n / L <= kite_safe_bpw(L, k)) {
+        if (usable(L, k, n) && (double)n / L <= kite_safe_bpw(L, k)) {
             lo = L;
             break;
         }
@@ -234,11 +269,11 @@ extern "C" uint32_t kite_choose_fft_len(uint64_t k, uint64_t n) {
     load_tune();
     bool all = true;
     for (uint32_t L = lo; L <= lo + lo / 10 * 3; L += 64)
-        if (usable(L) && tuned_us(L) < 0) all = false;
+        if (usable(L, k, n) && tuned_us(L) < 0) all = false;
     uint32_t best = lo;
     double bc = 1e300;
     for (uint32_t L = lo; L <= lo + lo / 10 * 3; L += 64) {
-        if (!usable(L)) continue;
+        if (!usable(L, k, n)) continue;
         // untuned: the radix cost model, with lengths that only the generic kernels run priced higher
         double c = all ? tuned_us(L) : cost_model(L) * (engine_shape_fast(L) ? 1.0 : 1.6);
         if (c < bc) {
@@ -359,9 +394,9 @@ void digits_to_limbs(kite_ctx *ctx, const int64_t *dig, uint64_t *limbs) {
 }
 
 void limbs_to_digits(kite_ctx *ctx, const uint64_t *limbs, int64_t *dig) {
-    size_t nl = ctx->nlimbs + 1;
-    std::vector<uint64_t> a(nl);
-    memcpy(a.data(), limbs, ctx->nlimbs * 8);
+    const size_t nl = ctx->nlimbs + 1;
+    std::vector<uint64_t> a(limbs, limbs + ctx->nlimbs);
+    a.push_back(0);
     // V = x*2^n mod N: x = q*k + r -> q + r*2^n
     u128 r = 0;
     for (size_t i = nl; i-- > 0;) {

@@ -394,6 +394,7 @@ SIGNATURES.update({
     "kite_batch_set_graphs": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_set_parts": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_set_xcds": (ctypes.c_int, [c_p, ctypes.c_int]),
+    "kite_debug_fail_hip": (None, [ctypes.c_long]),
     "kite_job_bytes": (c_u64, [c_u32, c_u32]),
     "kite_batch_run": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(RunOpts),
                                       ctypes.POINTER(JobResult)]),

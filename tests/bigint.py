@@ -82,3 +82,26 @@ def sqr_n(a, m, k, n):
     for _ in range(m):
         a = mod_riesel(mul(a, a), k, n)
     return a
+
+
+def lucas_v_matrix(k, n, P):
+    """V_{k 2^(n-1)}(P, -1) mod N = k*2^n-1 from the definition: (V_{m+1}, V_m) = M^m (P, 2) with
+    M = [[P, 1], [1, 0]] (V_{m+1} = P V_m + V_{m-1}), then V_2k = V_k^2 + 2 (k odd) and n - 2
+    doublings V_2m = V_m^2 - 2. O(log k) products, so k up to 2^30 costs nothing."""
+    N = k * (1 << n) - 1
+
+    def mat_mul(A, B):
+        return [[(A[0][0] * B[0][0] + A[0][1] * B[1][0]) % N, (A[0][0] * B[0][1] + A[0][1] * B[1][1]) % N],
+                [(A[1][0] * B[0][0] + A[1][1] * B[1][0]) % N, (A[1][0] * B[0][1] + A[1][1] * B[1][1]) % N]]
+
+    R, M, e = [[1, 0], [0, 1]], [[P, 1], [1, 0]], k
+    while e:
+        if e & 1:
+            R = mat_mul(R, M)
+        M = mat_mul(M, M)
+        e >>= 1
+    v = (R[1][0] * P + R[1][1] * 2) % N  # V_k: row 2 of M^k applied to (V_1, V_0)
+    v = (v * v + 2) % N
+    for _ in range(n - 2):
+        v = (v * v - 2) % N
+    return v

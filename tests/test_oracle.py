@@ -241,3 +241,10 @@ def test_fft_length_rule_matches_reference_fixture(golden):
     for k, n, _, L, _ in golden("large_fft_cases.csv"):
         assert O.lib().or_choose_fft_len(int(k), int(n)) == int(L)
     assert O.lib().or_choose_fft_len(9001, 662695) == 49152
+
+
+@pytest.mark.parametrize("k,n,P", [(1011, 61, 5), (3, 200, 3), (9357, 130, 7), (127, 97, 1)])
+def test_lucas_matrix_form_matches_recurrence(k, n, P):
+    """The O(log k) matrix form the large-k GPU tests use, against the plain recurrence."""
+    import bigint
+    assert bigint.lucas_v_matrix(k, n, P) == lucas_v(k, n, P)
