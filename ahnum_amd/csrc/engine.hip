@@ -331,11 +331,12 @@ static std::once_flag g_attr_once;
 
 static void set_lds_limits() {
     std::call_once(g_attr_once, [] {
-        PassKernel ks[256];
+        PassKernel ks[512];
         int n = 0;
-        all_kernels(ks, &n, 256);
+        all_kernels(ks, &n, 512);
         for (int i = 0; i < n; i++)
-            if (ks[i]) hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+            if (ks[i] && hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds) != hipSuccess)
+                (void)hipGetLastError();  // a refused limit must not surface at a later launch
     });
 }
 

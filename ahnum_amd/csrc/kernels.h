@@ -33,7 +33,7 @@ PassKernel fast_cols_pair_fwd(int R);
 PassKernel fast_cols_fused(int R);
 size_t fast_cols_fused_lds(int R);  // LDS bytes of the fused column kernel
 PassKernel fast_cols_pair_fused(int R);
-// every kernel pointer, for raising the LDS limit once
+// every kernel with dynamic LDS, for raising its LDS limit once
 void all_kernels(PassKernel *out, int *count, int cap);
 
 }  // namespace kite

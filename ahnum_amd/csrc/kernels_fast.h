@@ -1,0 +1,115 @@
+// Compile-time-size pass kernels (the bodies are the templates of fft_dev.h) and the list of column
+// lengths that have them. The column kernels are instantiated by kernels_cols_{1,2,3,4}.hip, one
+// slice of the list each, so that the slices compile in parallel; kernels_fast.hip holds the row
+// kernels and the registry of kernels.h.
+#ifndef KITE_KERNELS_FAST_H
+#define KITE_KERNELS_FAST_H
+
+#include "fft_dev.h"
+#include "kernels.h"
+
+// Column lengths R = 2^a m (m in 1, 3, 5, 7, 9, 15, 25, 27; 8 <= R <= 768, R divisible by 4) in
+// ascending order, as X(R, next R): a batch whose tests straddle two adjacent lengths runs both in
+// one launch of the pair kernel <R, next>.
+#define KITE_COLS_SLICE_1(X) \
+    X(8, 12) X(12, 16) X(16, 20) X(20, 24) X(24, 28) X(28, 32) X(32, 36) X(36, 40) X(40, 48) X(48, 56) X(56, 60) \
+    X(60, 64) X(64, 72) X(72, 80)
+#define KITE_COLS_SLICE_2(X) \
+    X(80, 96) X(96, 100) X(100, 108) X(108, 112) X(112, 120) X(120, 128) X(128, 144) X(144, 160) X(160, 192)
+#define KITE_COLS_SLICE_3(X) \
+    X(192, 200) X(200, 216) X(216, 224) X(224, 240) X(240, 256) X(256, 288) X(288, 320) X(320, 384)
+#define KITE_COLS_SLICE_4(X) \
+    X(384, 400) X(400, 432) X(432, 448) X(448, 480) X(480, 512) X(512, 576) X(576, 640) X(640, 768) X(768, 0)
+#define KITE_COLS_ALL(X) KITE_COLS_SLICE_1(X) KITE_COLS_SLICE_2(X) KITE_COLS_SLICE_3(X) KITE_COLS_SLICE_4(X)
+
+namespace kite {
+
+template <int C>
+__global__ void __launch_bounds__(kNT) k_rows_c(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+                                              int mode, int step) {
+    rows_fast<C>(slots, jobs, mode, step);
+}
+// One row pair per work-group: a single test's row pass on four times as many work-groups.
+template <int C>
+__global__ void __launch_bounds__(kNT) k_rows_c1(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+                                               int mode, int step) {
+    rows_fast<C, 1>(slots, jobs, mode, step);
+}
+template <int R>
+__global__ void __launch_bounds__(kNT) k_cols_inv_r(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+                                                  int mode, int step) {
+    cols_inv_fast<R>(slots, jobs, mode, step);
+}
+template <int R>
+__global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+                                                  int mode, int step) {
+    cols_fwd_fast<R>(slots, jobs, mode, step);
+}
+template <int R>
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_r(const SlotDev *__restrict__ slots,
+                                                       const int2 *__restrict__ jobs, int mode, int step) {
+    (void)mode;
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    cols_fused_fast<R>(slots[jb.x], jb.y, step);
+}
+
+// Two adjacent column lengths in one launch: tests of both shapes advance together (each work-
+// group runs the body of its test's R), so a batch straddling an FFT-length boundary fills the
+// device with one launch per column pass instead of two.
+template <int R1, int R2>
+__global__ void __launch_bounds__(kNT) k_cols_inv_rr(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+                                                   int mode, int step) {
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    if (slots[jb.x].R == R1) cols_inv_fast<R1>(slots, jobs, mode, step);
+    else cols_inv_fast<R2>(slots, jobs, mode, step);
+}
+template <int R1, int R2>
+__global__ void __launch_bounds__(kNT) k_cols_fwd_rr(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+                                                   int mode, int step) {
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    if (slots[jb.x].R == R1) cols_fwd_fast<R1>(slots, jobs, mode, step);
+    else cols_fwd_fast<R2>(slots, jobs, mode, step);
+}
+template <int R1, int R2>
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr(const SlotDev *__restrict__ slots,
+                                                        const int2 *__restrict__ jobs, int mode, int step) {
+    (void)mode;
+    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+    const SlotDev &s = slots[jb.x];
+    if (s.R == R1) cols_fused_fast<R1>(s, jb.y, step);
+    else cols_fused_fast<R2>(s, jb.y, step);
+}
+
+// The column kernels of one slice: kind is one of ColKind, R the (first) column length; nullptr if
+// the slice has no such kernel.
+enum ColKind : int { kColInv, kColFwd, kColFused, kColPairInv, kColPairFwd, kColPairFused };
+PassKernel cols_slice_1(int kind, int R);
+PassKernel cols_slice_2(int kind, int R);
+PassKernel cols_slice_3(int kind, int R);
+PassKernel cols_slice_4(int kind, int R);
+
+// A slice's lookup for one column length: the single-length kernels, and the pair kernels
+// <R, NEXT> when NEXT is nonzero.
+template <int R, int NEXT> PassKernel col_kernel(int kind) {
+    switch (kind) {
+    case kColInv: return (PassKernel)k_cols_inv_r<R>;
+    case kColFwd: return (PassKernel)k_cols_fwd_r<R>;
+    case kColFused: return (PassKernel)k_cols_fused_r<R>;
+    default: break;
+    }
+    if constexpr (NEXT != 0) {
+        switch (kind) {
+        case kColPairInv: return (PassKernel)k_cols_inv_rr<R, NEXT>;
+        case kColPairFwd: return (PassKernel)k_cols_fwd_rr<R, NEXT>;
+        case kColPairFused: return (PassKernel)k_cols_fused_rr<R, NEXT>;
+        default: break;
+        }
+    }
+    return nullptr;
+}
+#define KITE_COLS_CASE(R, NEXT) \
+    case R: return col_kernel<R, NEXT>(kind);
+
+}  // namespace kite
+
+#endif

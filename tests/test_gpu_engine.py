@@ -9,6 +9,7 @@ import pytest
 
 from ahnum_amd import kite, workloads
 import oracle as O
+import bigint
 
 pytestmark = pytest.mark.gpu
 
@@ -279,3 +280,60 @@ def test_llr_impossible_roundoff_limit():
     """A round-off limit nothing meets gives up with -2 (tests/test_llr.c:135-139)."""
     rc, r = kite.llr(3, 41628, max_roundoff=1e-9)
     assert rc == -2
+
+
+# column lengths with odd parts 9, 15, 25, 27 (compile-time kernels): the length whose preferred shape
+# has that column length (C = 4R or so), squared through the batch's fused column kernel
+ODD_PART_LENGTHS = [(36, 9216), (60, 30720), (72, 36864), (100, 51200), (108, 55296), (120, 122880),
+                    (144, 147456), (200, 409600), (216, 442368), (240, 491520), (288, 589824),
+                    (400, 1638400), (432, 1769472), (480, 1966080), (576, 2359296)]
+
+
+@pytest.mark.parametrize("R,L", ODD_PART_LENGTHS)
+def test_odd_part_column_lengths_exact(R, L):
+    """Two squarings at each new column length against exact integers, through its own
+    compile-time kernels (named in the launch timings)."""
+    k = 3
+    n = int(L * 13.5)
+    say("odd part", R, "L", L, "n", n)
+    r = random.Random(R)
+    b = kite.Batch()
+    try:
+        s = b.add(k, n, fft_len=L)
+        a = r.randrange(k * (1 << n) - 1)
+        b.set(s, 0, a)
+        b.timing(1)
+        b.square(2)
+        b.sync()
+        names = [name for name, *_ in b.kernel_times()]
+        say("  kernels", names)
+        assert any(name.endswith("<%d>" % R) for name in names), names
+        assert b.get(s, 0) == bigint.sqr_n(a, 2, k, n)
+        assert b.maxerr()[s] < 0.4
+    finally:
+        b.close()
+
+
+def test_adjacent_odd_part_lengths_share_launches():
+    """Tests at L = 49152 (R 96) and 51200 (R 100) in one batch (one stream, so that all three meet
+    in the same launches) run in the pair kernels <96, 100>."""
+    r = random.Random(96)
+    b = kite.Batch()
+    try:
+        b.parts(1)
+        vals = []
+        for k, n, L in [(8045, 659321, 49152), (9999, 670000, 51200), (3, 660001, 49152)]:
+            s = b.add(k, n, fft_len=L)
+            a = r.randrange(k * (1 << n) - 1)
+            b.set(s, 0, a)
+            vals.append((s, k, n, a))
+        b.timing(1)
+        b.square(2)
+        b.sync()
+        names = [name for name, *_ in b.kernel_times()]
+        say("kernels", names)
+        assert any("_rr<96,100>" in name for name in names), names
+        for s, k, n, a in vals:
+            assert b.get(s, 0) == bigint.sqr_n(a, 2, k, n), (k, n)
+    finally:
+        b.close()

@@ -42,15 +42,18 @@ def test_nm_exports_match_header():
         assert n in exported, n
 
 
-def test_fft_length_choice_is_safe_and_near_reference(golden):
+def test_fft_length_choice_matches_reference(golden):
+    """With compile-time column kernels for odd parts 1, 3, 5, 7, 9, 15, 25 and 27, and no tuning
+    file, the selector makes the reference's choice for its 7 production cases
+    (tests/large_fft_cases.csv:4-10, among them 294912 = 2^15 9 and 442368 = 2^14 27) and for the
+    record test (1638400 = 2^16 25, README.md:202)."""
     lib = kite.lib()
     for k, n, _, Lref, _ in golden("large_fft_cases.csv"):
         k, n, Lref = int(k), int(n), int(Lref)
         L = lib.kite_choose_fft_len(k, n)
-        assert L and n / L <= lib.kite_safe_bpw(L, k)
-        # the smallest safe length is the same rule; the pick among +30% follows this engine's shapes
-        assert L <= int(Lref * 1.3) + 64
+        assert L == Lref and n / L <= lib.kite_safe_bpw(L, k), (k, n)
     assert lib.kite_choose_fft_len(9001, 662695) == 49152
+    assert lib.kite_choose_fft_len(91, 24767982) == 1638400
 
 
 def test_next_fft_len():
