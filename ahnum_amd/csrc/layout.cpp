@@ -186,8 +186,11 @@ extern "C" double kite_weight(const kite_ctx *ctx, uint64_t j, uint64_t sj) {  /
 
 extern "C" double kite_safe_bpw(uint32_t L, uint64_t k) {
     // The reference's fitted model (src/ctx.c:26-33): 21.44 - 0.305 (log2 L - 12) - 0.5 log2 k
-    // less a 0.2 margin. The gfx950 engine's round-off stays below it at every length measured
-    // (tools: `kite sweep`), so the same bound is used.
+    // less a 0.2 margin. Refitted on the gfx950 engine (ahnum_amd/fit_bpw.py: the largest bits per
+    // word with round-off <= 0.25 over 300 squarings, L 4096..1048576, k 1..9999) it reads
+    // 21.389 - 0.306 (log2 L - 12) - 0.504 log2 k (rms 0.063): within 0.05 bits of the reference's,
+    // well inside the margin, and the round-off at this bound stays <= 0.25 at all 36 points. The
+    // reference's constants are kept, so that lengths are chosen as the reference chooses them.
     return 21.44 - 0.305 * (std::log2((double)L) - 12) - 0.50 * std::log2((double)k) - 0.2;
 }
 

@@ -107,3 +107,14 @@ def test_batch_run_refuses_an_unusable_checkpoint_dir(tmp_path):
         rc, out = kite.batch_run([(kite.JOB_PRP, 9357, 20000, 0.0), (kite.JOB_LLR, 3, 26459, 0.0)],
                                  checkpoint_dir=str(d))
         assert rc == -1 and [r.status for r in out] == [-1, -1]
+
+
+def test_bpw_fit_recovers_model_constants():
+    """The least-squares fit of ahnum_amd/fit_bpw.py returns the constants of limits generated from
+    the model itself (so a sign or column mix-up in the design matrix fails here)."""
+    from ahnum_amd import fit_bpw
+    import math
+    rows = [(L, k, 21.0 - 0.3 * (math.log2(L) - 12) - 0.55 * math.log2(k))
+            for L in (4096, 49152, 1048576) for k in (1, 3, 255, 9999)]
+    (A, B, C), rms = fit_bpw.fit(rows)
+    assert abs(A - 21.0) < 1e-9 and abs(B - 0.3) < 1e-9 and abs(C - 0.55) < 1e-9 and rms < 1e-9
