@@ -1359,9 +1359,10 @@ template <int R> struct FusedGeom {
     static constexpr int NE = R * G, NSC = R * NSUB, ITSC = (NSC + kNT - 1) / kNT, ITE = (NE + kNT - 1) / kNT;
     // LDS: R x G elements with row stride GP (the padding spreads the sub-chunk reads over the
     // banks), the column twiddles, one carry per sub-chunk, the group's weight factors (inverse and
-    // forward: 2G column and R row values each), a 64-digit scratch for the fold, the digit codes
+    // forward: 2G column and R row values each), a 64-digit scratch for the fold. (The digit codes
+    // stay in registers: at R = 96, G = 16 this is 31.5 KB, five work-groups per CU.)
     static constexpr size_t lds_bytes() {
-        return (size_t)16 * (R * GP + R) + (size_t)8 * NSC + (size_t)8 * (4 * G + 2 * R) + 4 * 64 + (size_t)R * G;
+        return (size_t)16 * (R * GP + R) + (size_t)8 * NSC + (size_t)8 * (4 * G + 2 * R) + 4 * 64;
     }
 };
 
@@ -1387,15 +1388,26 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     double *WB = reinterpret_cast<double *>(SC + NSC);  // [2G] inverse column, [2G] forward column
     double *WA = WB + 4 * G;                             // [R] inverse row (with k/(4H)), [R] forward row
     int32_t *FD = reinterpret_cast<int32_t *>(WA + 2 * R);
-    uint8_t *CD = reinterpret_cast<uint8_t *>(FD + 64);  // [R][G] digit codes
-    // The group's tables (column twiddles, weight factors, digit codes) are loaded into registers
-    // here and stored to LDS by pre() below, once the first FFT stage has issued its HBM loads:
-    // their round trips overlap instead of preceding the residue's.
-    constexpr int NTR = (R + kNT - 1) / kNT, NTB = (2 * G + kNT - 1) / kNT, NTC = (R * G / 4 + kNT - 1) / kNT;
+    // The group's tables (column twiddles, weight factors) are loaded into registers here and
+    // stored to LDS by pre() below, once the first FFT stage has issued its HBM loads: their round
+    // trips overlap instead of preceding the residue's.
+    constexpr int NTR = (R + kNT - 1) / kNT, NTB = (2 * G + kNT - 1) / kNT;
     double2 rT[NTR];
     double rWA[NTR], rWA2[NTR], rWB[NTB], rWB2[NTB];
-    uint32_t rCD[NTC];
-    const uint32_t *code4 = reinterpret_cast<const uint32_t *>(s.dcode + (size_t)g * R * G);
+    // The digit codes of the thread's own sub-chunks (SUB bytes each: row r, columns q SUB ..),
+    // fetched now and kept in registers for the carry and the weighting.
+    static_assert(SUB == 4 || SUB == 8, "digit codes are fetched as 4 or 8 bytes");
+    uint64_t cdw[ITSC];
+#pragma unroll
+    for (int it = 0; it < ITSC; it++) {
+        const int w = it * kNT + (int)threadIdx.x;
+        cdw[it] = 0;
+        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int q = w / R, r = w - q * R;
+        const uint8_t *cp = s.dcode + ((size_t)g * R + r) * G + q * SUB;
+        if constexpr (SUB == 8) cdw[it] = ldg(reinterpret_cast<const unsigned long long *>(cp));
+        else cdw[it] = ldg(reinterpret_cast<const uint32_t *>(cp));
+    }
 #pragma unroll
     for (int q = 0; q < NTR; q++) {
         const int i = q * kNT + (int)threadIdx.x;
@@ -1412,11 +1424,6 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
             rWB[q] = ldg(s.tb + 2 * C + 2 * c0 + i);
             rWB2[q] = ldg(s.tb + 2 * c0 + i);
         }
-    }
-#pragma unroll
-    for (int q = 0; q < NTC; q++) {
-        const int i = q * kNT + (int)threadIdx.x;
-        if ((R * G / 4) % kNT == 0 || i < R * G / 4) rCD[q] = ldg(code4 + i);
     }
     const double kscale = s.kscale;
     auto pre = [&]() {
@@ -1436,11 +1443,6 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
                 WB[i] = rWB[q];
                 WB[2 * G + i] = rWB2[q];
             }
-        }
-#pragma unroll
-        for (int q = 0; q < NTC; q++) {
-            const int i = q * kNT + (int)threadIdx.x;
-            if ((R * G / 4) % kNT == 0 || i < R * G / 4) reinterpret_cast<uint32_t *>(CD)[i] = rCD[q];
         }
     };
     const double2 *tg = s.twCols + (size_t)g * NE;
@@ -1470,7 +1472,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
         for (int l = 0; l < SUB; l++) {
             const int col = q * SUB + l;
             const double2 v = A[r * GP + col];
-            const int cd = CD[r * G + col];
+            const int cd = (int)(cdw[it] >> (8 * l)) & 0xFF;
             const double x0 = ldexp(v.x * (tap * WB[2 * col]), -(cd & 3));
             const double x1 = ldexp(v.y * (tap * WB[2 * col + 1]), -((cd >> 3) & 3));
             const double r0 = rint(x0), r1 = rint(x1);
@@ -1536,7 +1538,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
 #pragma unroll
         for (int l = 0; l < SUB; l++) {
             const int col = q * SUB + l;
-            const int cd = CD[r * G + col];
+            const int cd = (int)(cdw[it] >> (8 * l)) & 0xFF;
             const double w0 = ldexp(tam * WB[2 * G + 2 * col], cd & 3);
             const double w1 = ldexp(tam * WB[2 * G + 2 * col + 1], (cd >> 3) & 3);
             A[r * GP + col] = make_double2((double)d[it][2 * l] * w0, (double)d[it][2 * l + 1] * w1);

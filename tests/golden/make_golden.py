@@ -4,6 +4,9 @@
   python tests/golden/make_golden.py plain_small     # 16 seeded n~65K tests: RES64 by GMP
   python tests/golden/make_golden.py chain           # x_i = 3^(k*2^i) mod N at chosen i
   python tests/golden/make_golden.py batch_subset N  # first N tests of the 256-test batch
+  python tests/golden/make_golden.py batch_more      # the batch's other tests, smallest n first,
+                                                     # each appended to _bin/batch_more.partial
+  python tests/golden/make_golden.py batch_merge     # res64_batch_subset.csv += finished rows
   python tests/golden/make_golden.py single_large    # the seeded n~660K test
 
 GMP (mpz_powm, the semantics of the reference's tools/gmp_ref.c) runs through gmp_res64.c,
@@ -54,6 +57,11 @@ def write(name, header, rows):
     print("wrote", path, len(rows), "rows")
 
 
+def read_rows(name):
+    path = os.path.join(HERE, name)
+    return [l.strip() for l in open(path) if l.strip() and not l.startswith("#")]
+
+
 def main():
     what = sys.argv[1]
     jobs = int(os.environ.get("GOLDEN_JOBS", "4"))
@@ -78,6 +86,21 @@ def main():
         rows = gmp_lines([f"{k} {n} 3" for k, n in tests], jobs,
                          partial=os.path.join(HERE, "_bin", "batch_subset.partial"))
         write("res64_batch_subset.csv", f"# k,n,base,RES64 (GMP): first {cnt} of workloads.batch_mixed(256, seed=3)", rows)
+    elif what == "batch_more":
+        have = {tuple(r.split(",")[:2]) for r in read_rows("res64_batch_subset.csv")}
+        todo = sorted((n, k) for k, n in workloads.batch_mixed() if (str(k), str(n)) not in have)
+        gmp_lines([f"{k} {n} 3" for n, k in todo], jobs, partial=os.path.join(HERE, "_bin", "batch_more.partial"))
+    elif what == "batch_merge":
+        rows = {tuple(r.split(",")[:2]): r for r in read_rows("res64_batch_subset.csv")}
+        part = os.path.join(HERE, "_bin", "batch_more.partial")
+        if os.path.exists(part):
+            for line in open(part):
+                if line.strip():
+                    rows[tuple(line.split(",")[:2])] = line.strip()
+        order = [(str(k), str(n)) for k, n in workloads.batch_mixed()]
+        out = [rows[key] for key in order if key in rows]
+        write("res64_batch_subset.csv", f"# k,n,base,RES64 (GMP): {len(out)} of the 256 tests of "
+              "workloads.batch_mixed(256, seed=3), in batch order", out)
     elif what == "single_large":
         rows = gmp_lines([f"{k} {n} 3" for k, n in workloads.single_large()], 1,
                          partial=os.path.join(HERE, "_bin", "single_large.partial"))
