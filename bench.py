@@ -140,6 +140,7 @@ def cpu_baseline(tests, secs):
 # ---- traffic summary ---------------------------------------------------------------------------
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_large_batch.txt")
+TUNED_FILE = os.path.join(ROOT, "ahnum_amd", "tuned", "single_large.json")
 
 
 def load_traffic(kernel):
@@ -270,7 +271,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--batch", type=int, default=384, help="tests of the single_large workload")
+    ap.add_argument("--batch", type=int, default=None,
+                    help="tests of the single_large workload (default: the tuned value, else 384)")
+    ap.add_argument("--tuned", default="auto",
+                    help="batched autotuner result (python -m ahnum_amd.tune_batch) applied to the single_large "
+                         "batch: shape variant per length, streams, XCD layout, batch size; 'auto' takes "
+                         "ahnum_amd/tuned/single_large.json when present, 'none' runs the defaults")
     ap.add_argument("--workload", default="single_large", choices=sorted(WORKLOADS),
                     help="batch workload (single_large: the n~660K batch the metric is quoted on)")
     ap.add_argument("--plain-steps", type=int, default=10000, help="squarings of the one-test microbench")
@@ -285,7 +291,10 @@ def main():
                     help="bracket the launches of every N-th step of the kernel-timing region with HIP events")
     ap.add_argument("--time-steps", type=int, default=1000, help="steps of the kernel-timing region")
     ap.add_argument("--block", type=int, default=1000, help="squarings per square call (a Gerbicz block)")
-    ap.add_argument("--parts", type=int, default=2, help="concurrent streams the batch is dealt to")
+    ap.add_argument("--parts", type=int, default=None,
+                    help="concurrent streams the batch is dealt to (default: the tuned value, else 2)")
+    ap.add_argument("--xcds", type=int, default=None,
+                    help="XCDs the column jobs are laid out for (default: the tuned value, else 8)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a fill, a copy and an add over this many MiB (calibrates the HBM "
                          "byte counters of a rocprofv3 --pmc run of this command)")
@@ -313,6 +322,26 @@ def main():
     if a.serve:
         serve_only(a)
         return
+
+    # the batched autotuner's settings; the shape variants reach the library through its tuning
+    # file, which it reads once, so they are set before it loads
+    tuned_path, tuned = None, {}
+    if a.workload == "single_large" and a.tuned != "none":
+        tuned_path = TUNED_FILE if a.tuned == "auto" else a.tuned
+        if os.path.exists(tuned_path):
+            with open(tuned_path) as f:
+                tuned = json.load(f)["best"]
+            if tuned.get("orders"):
+                from ahnum_amd import tune_batch
+                os.environ["KITE_FFT_TIMES"] = tune_batch.fft_times_file(tuned["orders"])
+        else:
+            tuned_path = None
+    if a.batch is None:
+        a.batch = int(tuned.get("batch", 384))
+    if a.parts is None:
+        a.parts = int(tuned.get("parts", 2))
+    if a.xcds is None:
+        a.xcds = int(tuned.get("xcds", 8))
 
     from ahnum_amd import kite, workloads
 
@@ -353,6 +382,7 @@ def main():
     # ---- batched engine ----
     b = kite.Batch()
     parts = b.parts(a.parts)
+    b.xcds(a.xcds)
     Ls = []
     for i, (k, n) in enumerate(tests):
         s = b.add(k, n)
@@ -420,7 +450,9 @@ def main():
             "workload": described,
             "batch": len(tests),
             "fft_lengths": sorted(set(Ls)),
-            "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams",
+            "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams, "
+                      f"column jobs laid out for {a.xcds} XCDs",
+            "tuned": ({"file": os.path.relpath(tuned_path, ROOT), **tuned} if tuned_path else None),
             "value_is": "squaring throughput of the timed steps as tests/hour (sum of squarings/s * 3600 / n); "
                         "the production loop with Gerbicz checks is timed apart in gerbicz_checked",
             "seed": seed,
