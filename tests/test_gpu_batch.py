@@ -45,7 +45,7 @@ def test_mixed_lengths_one_launch():
 
 
 @pytest.mark.parametrize("cases", [
-    # the n~660K boundary: L 49152 (R 96) and 57344 (R 112) share one launch per column pass
+    # the n~660K boundary: L 49152 (R 96) and 51200 (R 100) share one launch per column pass
     [(8045, 659321, 0), (9001, 669990, 0), (3, 658000, 0), (9999, 669000, 0)],
     # small adjacent column lengths from explicit FFT lengths
     [(9357, 20000, 0), (4523, 20011, 2560), (3, 20000, 3072), (7, 19000, 3584)],

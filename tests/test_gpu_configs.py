@@ -1,7 +1,7 @@
 """The large-FFT batch and the serving stream of the benchmark configurations, on the GPU.
 
   large-FFT batch: 32 Fermat base-3 tests, k odd in [3, 999], n in [3M, 6M] (workloads.large_fft),
-                   FFT 229376-458752, as one batch through the shared launches (two adjacent column
+                   FFT 204800-442368, as one batch through the shared launches (two adjacent column
                    lengths per launch): two squarings of every slot against exact integer squaring,
                    then the production loop (Gerbicz blocks and checks) for a bounded time; and the
                    reference's three smallest large-FFT residues (tests/large_fft_cases.csv) to

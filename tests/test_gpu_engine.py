@@ -98,8 +98,8 @@ def test_arith_vs_exact_ints(k, n, L):
 
 
 @pytest.mark.parametrize("k,n,m", [
-    (883, 4046207, 4), (109, 5665875, 3), (497, 5902972, 3),  # the 3M-6M configuration (L 327680-458752)
-    (91, 24767982, 2),  # record size, L 1835008
+    (883, 4046207, 4), (109, 5665875, 3), (497, 5902972, 3),  # the 3M-6M configuration (L 294912-409600)
+    (91, 24767982, 2),  # record size, L 1638400
 ])
 def test_large_lengths_vs_exact_ints(k, n, m):
     """Squarings at the large FFT lengths the selector picks for n = 3M..25M, checked against exact
