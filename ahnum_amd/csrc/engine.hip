@@ -862,6 +862,11 @@ struct kite_batch {
         int stat_f = -1;
         size_t lds_f = 0;
         int part = 0;  // the stream the group's launches go to
+        // the persistent variant of the square-mode pass (rows: k2, columns: kf), if any; its job
+        // lists (index into d_lists, the list count, the longest list) and the work-groups each list
+        // needs at least (kernels_fast.h: persist_loop)
+        kite::PersistKernel kp = nullptr;
+        int ls = -1, nl = 1, maxcnt = 0, pmin = 1, stat_p = -1;
     };
     std::vector<Group> grows, gcols;
     // The slots are split into `nparts` parts with a stream each: the parts' launches run
@@ -884,6 +889,13 @@ struct kite_batch {
     // column jobs are laid out for this many XCDs (work-group i runs on XCD i mod xcds): each
     // test's column groups go to one XCD, in order (1: plain order)
     int xcds = kite::kXcds;
+    // persistent launches (kite_batch_set_persistent): square-mode passes with a persistent kernel
+    // run on at most the device's resident capacity (divided among the parts), each work-group
+    // running a fixed share of the jobs
+    int persist = 0;
+    unsigned int *d_lists = nullptr;  // kPersistListWords per group with a persistent kernel
+    size_t lists_cap = 0;
+    int2 *d_cols_p = nullptr;         // column jobs list by list (the persistent launches' order)
     int timing = 0;  // 0: off; N: every N-th step of a square call is timed
     // timing: one event pair per launch of the last timed call, and what each pair measured
     std::vector<hipEvent_t> ev;
@@ -973,8 +985,9 @@ static int batch_refresh(kite_batch *b) {
         }
         b->slot_cap = nc;
     }
-    // jobs grouped by kernel (one launch per group), groups contiguous in the job arrays
-    std::vector<int2> rows, cols;
+    // jobs grouped by kernel (one launch per group), groups contiguous in the job arrays; cols_orig:
+    // the column jobs before the XCD layout (each test's jobs consecutive)
+    std::vector<int2> rows, cols, cols_orig;
     b->grows.clear();
     b->gcols.clear();
     std::vector<int> order;
@@ -1036,11 +1049,15 @@ static int batch_refresh(kite_batch *b) {
             if (isrows) {
                 g.stat[0] = stat_of(rk0 == E0->krows1 && alone ? "k_rows_c1<" + std::to_string(E0->shape->ch.C) + ">"
                                                                : E0->name[0], 0);
+                g.kp = rsq0 ? fast_rows_sq_p(E0->shape->ch.C) : nullptr;
+                if (g.kp) g.stat_p = stat_of(E0->name[0] + "_p", 0);
             } else {
                 g.stat[0] = stat_of(E0->name[1], 1);
                 g.stat[1] = stat_of(E0->name[2], 2);
                 g.kf = E0->kcfused;
                 if (g.kf) g.stat_f = stat_of("k_cols_fused_r<" + std::to_string(E0->shape->ch.R) + ">", 1);
+                g.kp = g.kf ? fast_cols_fused_p(E0->shape->ch.R) : nullptr;
+                if (g.kp) g.stat_p = stat_of("k_cols_fused_r_p<" + std::to_string(E0->shape->ch.R) + ">", 1);
             }
             g.part = part;
             grp.push_back(g);
@@ -1057,6 +1074,9 @@ static int batch_refresh(kite_batch *b) {
         // exchange their flags through that XCD's L2. Each test's jobs keep their relative order,
         // so every flag a work-group waits for is still published by one dispatched before it or
         // by its own test's last group, as before.
+        cols_orig.resize(jobs.size());
+        for (const kite_batch::Group &g : grp)
+            std::copy(jobs.begin() + g.off, jobs.begin() + g.off + g.count, cols_orig.begin() + g.off);
         if (b->xcds > 1)
             for (const kite_batch::Group &g : grp) {
                 std::vector<std::vector<int2>> lists(b->xcds);
@@ -1096,6 +1116,8 @@ static int batch_refresh(kite_batch *b) {
                 g.stat[1] = stat_of("k_cols_fwd_rr" + rr, 2);
                 g.kf = (grp[i].kf && grp[i + 1].kf) ? fast_cols_pair_fused(r1) : nullptr;
                 g.stat_f = g.kf ? stat_of("k_cols_fused_rr" + rr, 1) : -1;
+                g.kp = g.kf ? fast_cols_pair_fused_p(r1) : nullptr;
+                g.stat_p = g.kp ? stat_of("k_cols_fused_rr_p" + rr, 1) : -1;
                 merged.push_back(g);
                 i++;
             } else {
@@ -1108,16 +1130,74 @@ static int batch_refresh(kite_batch *b) {
         build(true, part);
         build(false, part);
     }
+    // Persistent launches: the job lists of each group that has a persistent kernel (each list's
+    // first job and count), and the column jobs list by list, whole tests dealt round-robin to
+    // min(xcds, tests, 8) lists.
+    std::vector<int2> cols_p(cols);
+    std::vector<uint32_t> lsh;
+    auto add_lists = [&](kite_batch::Group &g, int nl, const uint32_t *off, const uint32_t *cnt, int pmin) {
+        g.ls = (int)(lsh.size() / kPersistListWords);
+        g.nl = nl;
+        g.maxcnt = 0;
+        lsh.resize(lsh.size() + kPersistListWords, 0);
+        uint32_t *t = &lsh[(size_t)g.ls * kPersistListWords];
+        for (int x = 0; x < nl; x++) {
+            t[kLsOff + x] = off[x];
+            t[kLsCnt + x] = cnt[x];
+            g.maxcnt = std::max(g.maxcnt, (int)cnt[x]);
+        }
+        g.pmin = std::min(pmin, g.maxcnt);
+    };
+    for (kite_batch::Group &g : b->grows)
+        if (g.kp) {
+            const uint32_t off = 0, cnt = (uint32_t)g.count;
+            add_lists(g, 1, &off, &cnt, 1);
+        }
+    for (kite_batch::Group &g : b->gcols) {
+        if (!g.kp) continue;
+        std::vector<std::pair<int, int>> runs;  // each test's jobs in cols_orig: (first, count)
+        int ng = 1;
+        for (int i = g.off; i < g.off + g.count;) {
+            int j = i;
+            while (j < g.off + g.count && cols_orig[j].x == cols_orig[i].x) j++;
+            runs.push_back({i, j - i});
+            ng = std::max(ng, j - i);
+            i = j;
+        }
+        const int nl = std::max(1, std::min({b->xcds, kXcds, (int)runs.size()}));
+        uint32_t off[kXcds], cnt[kXcds];
+        int pos = g.off;
+        for (int x = 0; x < nl; x++) {
+            off[x] = (uint32_t)(pos - g.off);
+            for (size_t t = (size_t)x; t < runs.size(); t += (size_t)nl)
+                for (int u = 0; u < runs[t].second; u++) cols_p[pos++] = cols_orig[runs[t].first + u];
+            cnt[x] = (uint32_t)(pos - g.off) - off[x];
+        }
+        add_lists(g, nl, off, cnt, ng);  // a column job waits for jobs up to NG - 1 positions later
+    }
     const size_t need = std::max(rows.size(), cols.size());
     if (need > b->jobs_cap) {
         if (b->d_rows) hipFree(b->d_rows);
         if (b->d_cols) hipFree(b->d_cols);
-        b->d_rows = b->d_cols = nullptr;
+        if (b->d_cols_p) hipFree(b->d_cols_p);
+        b->d_rows = b->d_cols = b->d_cols_p = nullptr;
         const size_t nc = std::max(need, 2 * b->jobs_cap);
         BATCH_OK(b, hipMalloc(&b->d_rows, nc * sizeof(int2)));
         BATCH_OK(b, hipMalloc(&b->d_cols, nc * sizeof(int2)));
+        BATCH_OK(b, hipMalloc(&b->d_cols_p, nc * sizeof(int2)));
         b->jobs_cap = nc;
     }
+    if (lsh.size() > b->lists_cap) {
+        if (b->d_lists) hipFree(b->d_lists);
+        b->d_lists = nullptr;
+        const size_t nc = std::max(lsh.size(), 2 * b->lists_cap);
+        BATCH_OK(b, hipMalloc(&b->d_lists, nc * sizeof(uint32_t)));
+        b->lists_cap = nc;
+    }
+    if (!lsh.empty())
+        BATCH_OK(b, hipMemcpy(b->d_lists, lsh.data(), lsh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!cols_p.empty())
+        BATCH_OK(b, hipMemcpy(b->d_cols_p, cols_p.data(), cols_p.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!rows.empty()) BATCH_OK(b, hipMemcpy(b->d_rows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!cols.empty()) BATCH_OK(b, hipMemcpy(b->d_cols, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
     b->nrows = (int)rows.size();
@@ -1164,6 +1244,32 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
     return 0;
 }
 
+// Work-groups of a persistent launch (0: launch one work-group per job instead): at most the
+// kernel's resident capacity on the device (the occupancy its registers and LDS allow, times the
+// CUs) divided among the parts whose launches run at the same time, nl lists of P work-groups each,
+// P no more than the longest list and at least what the lists' waits need (persist_loop).
+static int persist_grid(kite_batch *b, const kite_batch::Group &g, int threads, size_t lds) {
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<const void *, size_t>, int>> cache;  // (kernel, lds) -> WGs resident
+    int cap = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const auto &c : cache)
+            if (c.first.first == g.kp && c.first.second == lds) cap = c.second;
+        if (!cap) {
+            int per_cu = 0, dev = 0, cus = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.kp, threads, lds) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                return 0;
+            cap = std::max(per_cu, 0) * std::max(cus, 0);
+            cache.push_back({{g.kp, lds}, cap});
+        }
+    }
+    const int P = std::min(cap / std::max(1, b->nparts) / g.nl, g.maxcnt);
+    return P >= g.pmin && P > 0 ? P * g.nl : 0;
+}
+
 // One launch per group of the pass; with `timed`, each launch is bracketed by an event pair.
 // In square mode a column group with a fused kernel runs it as pass 1 and skips pass 2.
 static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool timed = false) {
@@ -1177,6 +1283,16 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
         const PassKernel fn = fused ? g.kf : (pass == 2 || rsq) ? g.k2 : g.k1;
         const size_t lds = rsq ? 0 : fused ? g.lds_f : g.lds;
         const int threads = rsq ? kRowsSqThreads : kNT;
+        const int pgrid = b->persist && g.kp && (rsq || fused) ? persist_grid(b, g, threads, lds) : 0;
+        const bool persist = pgrid > 0;
+        auto launch = [&]() -> hipError_t {
+            if (!persist) return launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, jobs + g.off, mode, step, threads);
+            const int2 *pj = (pass == 0 ? b->d_rows : b->d_cols_p) + g.off;
+            const unsigned int *ls = b->d_lists + (size_t)g.ls * kPersistListWords;
+            int nl = g.nl;
+            void *args[] = {(void *)&b->d_slots, (void *)&pj, (void *)&step, (void *)&ls, (void *)&nl};
+            return hipLaunchKernel(g.kp, dim3(pgrid), dim3(threads), args, lds, b->ps[g.part]);
+        };
         hipError_t e;
         if (timed) {
             const size_t ei = 2 * b->ev_stat.size();
@@ -1185,12 +1301,12 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
                 hipEventCreate(&b->ev[ei]);
                 hipEventCreate(&b->ev[ei + 1]);
             }
-            b->ev_stat.push_back(fused ? g.stat_f : g.stat[pass == 2 ? 1 : 0]);
+            b->ev_stat.push_back(persist ? g.stat_p : fused ? g.stat_f : g.stat[pass == 2 ? 1 : 0]);
             hipEventRecord(b->ev[ei], b->ps[g.part]);
-            e = launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, jobs + g.off, mode, step, threads);
+            e = launch();
             hipEventRecord(b->ev[ei + 1], b->ps[g.part]);
         } else {
-            e = launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, jobs + g.off, mode, step, threads);
+            e = launch();
         }
         if (e != hipSuccess) return e;
     }
@@ -1225,6 +1341,8 @@ extern "C" void kite_batch_free(kite_batch *b) {
     if (b->h_slots) hipHostFree(b->h_slots);
     if (b->d_rows) hipFree(b->d_rows);
     if (b->d_cols) hipFree(b->d_cols);
+    if (b->d_cols_p) hipFree(b->d_cols_p);
+    if (b->d_lists) hipFree(b->d_lists);
     if (b->d_maxerr) hipFree(b->d_maxerr);
     if (b->d_cvterr) hipFree(b->d_cvterr);
     if (b->d_eq) hipFree(b->d_eq);
@@ -1391,12 +1509,16 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
         for (auto &ks : b->kstats) ks = kite_batch::KStat{ks.name, ks.pass};
         std::vector<double> bytes(b->kstats.size(), 0.0);
         for (const auto &g : b->grows)
-            if (g.count) bytes[g.stat[0]] += 16.0 * g.sumL;
+            if (g.count) {
+                bytes[g.stat[0]] += 16.0 * g.sumL;
+                if (g.kp) bytes[g.stat_p] += 16.0 * g.sumL;
+            }
         for (const auto &g : b->gcols)
             if (g.count) {
                 bytes[g.stat[0]] += 8.0 * g.sumL;
                 bytes[g.stat[1]] += 8.0 * g.sumL;
                 if (g.kf) bytes[g.stat_f] += 16.0 * g.sumL;
+                if (g.kp) bytes[g.stat_p] += 16.0 * g.sumL;
             }
         for (size_t i = 0; i < b->ev_stat.size(); i++) {
             float ms = 0;
@@ -1488,6 +1610,15 @@ extern "C" int kite_batch_set_xcds(kite_batch *b, int xcds) {
         b->dirty = true;
     }
     return xcds;
+}
+
+extern "C" int kite_batch_set_persistent(kite_batch *b, int on) {
+    on = on != 0;
+    if (on != b->persist) {
+        b->persist = on;
+        b->dirty = true;  // the captured graph holds the other launches
+    }
+    return on;
 }
 
 extern "C" int kite_batch_set_graphs(kite_batch *b, int on) {

@@ -716,6 +716,15 @@ __device__ __forceinline__ void cols_fwd_body(const SlotDev *__restrict__ slots,
 // in registers and moving its digits straight between HBM and registers, the full w_H table.
 // ================================================================================================
 
+// The work-item id through an opaque copy: in a kernel that loops over jobs, everything derived
+// from the id would otherwise be hoisted out of the loop as invariant and kept live across the
+// whole job body (which spilled hundreds of bytes per thread); each job reads its own copy.
+__device__ __forceinline__ int tid_x() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // LDS index with PAD: one padding element after every 2^PAD (PAD = 0: none), which spreads the
 // strided writes of the early Stockham stages over distinct LDS bank groups.
 template <int PAD> __device__ __forceinline__ int padi(int i) { return PAD ? i + (i >> PAD) : i; }
@@ -785,6 +794,7 @@ template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT
 __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST &st, PRE pre = PRE()) {
     constexpr int NST = io_nstages(N);
     if constexpr (I < NST) {
+        const int tid = tid_x();
         constexpr int P = io_radix(N, I), NB = N / P, TOTAL = NB * HOW;
         constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
         constexpr bool FROM_LD = LDIN && I == 0, TO_ST = STOUT && I == NST - 1;
@@ -792,7 +802,7 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
         int orow[ITER], seqs[ITER];
 #pragma unroll
         for (int it = 0; it < ITER; it++) {
-            const int w = it * kNT + (int)threadIdx.x;
+            const int w = it * kNT + tid;
             if (TOTAL % kNT == 0 || w < TOTAL) {
                 const int seq = w % HOW, j = w / HOW, jm = j % NS;
 #pragma unroll
@@ -815,7 +825,7 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
         if constexpr (TO_ST) {
 #pragma unroll
             for (int it = 0; it < ITER; it++) {
-                const int w = it * kNT + (int)threadIdx.x;
+                const int w = it * kNT + tid;
                 if (TOTAL % kNT == 0 || w < TOTAL) {
 #pragma unroll
                     for (int r = 0; r < P; r++) st(it, r, orow[it] + r * NS, seqs[it], v[it][r]);
@@ -825,7 +835,7 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
             if constexpr (!FROM_LD) __syncthreads();  // the stage is in place: reads before writes
 #pragma unroll
             for (int it = 0; it < ITER; it++) {
-                const int w = it * kNT + (int)threadIdx.x;
+                const int w = it * kNT + tid;
                 if (TOTAL % kNT == 0 || w < TOTAL) {
 #pragma unroll
                     for (int r = 0; r < P; r++) a[(orow[it] + r * NS) * IS + seqs[it]] = v[it][r];
@@ -1024,10 +1034,9 @@ __device__ __forceinline__ void xpose_ri(double2 *v, double *X, WR wr, RD rd) {
     __syncthreads();
 }
 
-__device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs, int step) {
+__device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, int2 jb, int step) {
     constexpr int C = kRowsRegC;
     __shared__ double xs[8 * 256];  // one 2 KB block per 16-lane group
-    const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     const SlotDev &s = slots[jb.x];
     {
         // the slot fields of the prologue in one scalar round trip
@@ -1037,7 +1046,7 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, co
         asm volatile("" ::"s"(count), "s"(R0), "s"(p0), "s"(p1), "s"(p2));
         if (step >= (int)count) return;
     }
-    const int lane = (int)threadIdx.x, g = lane >> 4, t = lane & 15;
+    const int lane = tid_x(), g = lane >> 4, t = lane & 15;
     const int side = g >> 2, p = jb.y * 4 + (g & 3);
     const int R = s.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
     const bool special = p == 0;  // rows 0 and R/2, each paired with itself
@@ -1381,6 +1390,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
         asm volatile("" ::"s"(count), "s"(C0), "s"(p0), "s"(p1), "s"(p2), "s"(p3), "s"(p4), "s"(p5));
         if (step >= (int)count) return;
     }
+    const int tid = tid_x();
     const int C = s.C, NG = C / G, c0 = g * G;
     const uint32_t seq = s.seq + (uint32_t)step + 1;
     double2 *A = lds, *T = lds + R * GP;
@@ -1400,7 +1410,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     uint64_t cdw[ITSC];
 #pragma unroll
     for (int it = 0; it < ITSC; it++) {
-        const int w = it * kNT + (int)threadIdx.x;
+        const int w = it * kNT + tid;
         cdw[it] = 0;
         if (NSC % kNT != 0 && w >= NSC) continue;
         const int q = w / R, r = w - q * R;
@@ -1410,7 +1420,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     }
 #pragma unroll
     for (int q = 0; q < NTR; q++) {
-        const int i = q * kNT + (int)threadIdx.x;
+        const int i = q * kNT + tid;
         if (R % kNT == 0 || i < R) {
             rT[q] = gld(s.twR + i);
             rWA[q] = ldg(s.ta + R + i);
@@ -1419,7 +1429,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     }
 #pragma unroll
     for (int q = 0; q < NTB; q++) {
-        const int i = q * kNT + (int)threadIdx.x;
+        const int i = q * kNT + tid;
         if ((2 * G) % kNT == 0 || i < 2 * G) {
             rWB[q] = ldg(s.tb + 2 * C + 2 * c0 + i);
             rWB2[q] = ldg(s.tb + 2 * c0 + i);
@@ -1429,7 +1439,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     auto pre = [&]() {
 #pragma unroll
         for (int q = 0; q < NTR; q++) {
-            const int i = q * kNT + (int)threadIdx.x;
+            const int i = q * kNT + tid;
             if (R % kNT == 0 || i < R) {
                 T[i] = rT[q];
                 WA[i] = rWA[q] * kscale;
@@ -1438,7 +1448,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
         }
 #pragma unroll
         for (int q = 0; q < NTB; q++) {
-            const int i = q * kNT + (int)threadIdx.x;
+            const int i = q * kNT + tid;
             if ((2 * G) % kNT == 0 || i < 2 * G) {
                 WB[i] = rWB[q];
                 WB[2 * G + i] = rWB2[q];
@@ -1460,7 +1470,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     uint32_t bg[ITSC];  // bit i: digit i of the sub-chunk has b0+1 bits
 #pragma unroll
     for (int it = 0; it < ITSC; it++) {
-        const int w = it * kNT + (int)threadIdx.x;
+        const int w = it * kNT + tid;
         if (NSC % kNT != 0 && w >= NSC) continue;
         // sub-chunk (r, q) is w = q R + r: the lanes of a wave take consecutive rows, so their
         // reads of A (row stride G + 1) fall in distinct LDS banks
@@ -1502,7 +1512,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     __syncthreads();
     // the top fold, here where few registers are live (its 128-bit arithmetic inside the loop above
     // made every thread spill); the last sub-chunk's carry goes nowhere inside the work-group
-    if (g == NG - 1 && (int)threadIdx.x == (NSC - 1) % kNT) {
+    if (g == NG - 1 && tid == (NSC - 1) % kNT) {
         constexpr int it = ITSC - 1;
         const int64_t co = fold_top_dev(s, FD, SC[NSC - 1]);
         SC[NSC - 1] = co;
@@ -1515,7 +1525,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     // flag); the last digit keeps the rest. Then weight into A for the forward column FFT.
 #pragma unroll
     for (int it = 0; it < ITSC; it++) {
-        const int w = it * kNT + (int)threadIdx.x;
+        const int w = it * kNT + tid;
         if (NSC % kNT != 0 && w >= NSC) continue;
         const int q = w / R, r = w - q * R;
         int64_t cy;
@@ -1556,7 +1566,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     asm volatile("" : "+v"(tgs), "+v"(xo), "+v"(Co));
 #pragma unroll
     for (int it = 0; it < LS::ITER; it++) {
-        const int w = it * kNT + (int)threadIdx.x;
+        const int w = it * kNT + tid;
         if (LS::TOTAL % kNT == 0 || w < LS::TOTAL) {
 #pragma unroll
             for (int r = 0; r < LS::P; r++) tws[it][r] = gld(tgs + (w / G + r * LS::NB) * G + w % G);
@@ -1569,7 +1579,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
         fft_io<R, 1, 0, G, GP, false, false, true>(A, T, ld, st);
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
-    if ((threadIdx.x & 63) == 0 && lmax > 0)
+    if ((tid & 63) == 0 && lmax > 0)
         atomicMax(s.maxerr, (unsigned long long)__double_as_longlong(lmax));
 }
 

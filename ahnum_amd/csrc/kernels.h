@@ -10,6 +10,11 @@
 namespace kite {
 
 typedef const void *PassKernel;  // a __global__ function (const SlotDev *slots, const int2 *jobs, int mode, int step)
+// a persistent square-mode pass: (const SlotDev *slots, const int2 *jobs, int step, const unsigned
+// int *lists, int nl); kernels_fast.h describes the job lists and which work-group runs which job
+typedef const void *PersistKernel;
+constexpr int kPersistListWords = 16;
+constexpr int kLsOff = 0, kLsCnt = 8;  // words of a launch's job lists: first job, count (8 lists)
 
 PassKernel generic_rows();
 PassKernel generic_cols_inv();
@@ -33,6 +38,10 @@ PassKernel fast_cols_pair_fwd(int R);
 PassKernel fast_cols_fused(int R);
 size_t fast_cols_fused_lds(int R);  // LDS bytes of the fused column kernel
 PassKernel fast_cols_pair_fused(int R);
+// persistent variants (nullptr: none): the register row kernel, the fused column kernels
+PersistKernel fast_rows_sq_p(int C);
+PersistKernel fast_cols_fused_p(int R);
+PersistKernel fast_cols_pair_fused_p(int R);
 // every kernel with dynamic LDS, for raising its LDS limit once
 void all_kernels(PassKernel *out, int *count, int cap);
 

@@ -80,9 +80,57 @@ __global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr(const SlotDev *__restr
     else cols_fused_fast<R2>(s, jb.y, step);
 }
 
+// ---- persistent launches ----------------------------------------------------------------------
+// A persistent launch runs no more work-groups than the device holds at once, each running a fixed
+// share of the jobs in turn, instead of one work-group per job: the dispatcher no longer has to
+// refill the CUs as work-groups finish (with one work-group per job about a fifth of the resident
+// slots stand empty between a work-group's end and the next one's start). Jobs are not handed out
+// by atomic tickets: same-address device-scope atomics serialize (tens of ns each), which cost
+// more than the empty slots.
+//
+// Job lists (kPersistListWords words per launch, written by the host): words 0..7 each list's
+// first job, 8..15 its job count. The jobs are laid out list by list, each test's jobs consecutive
+// within one list (the column jobs' XCD lists); the launch has nl * P work-groups and work-group
+// w = x + nl k runs positions k, k + P, k + 2P, ... of list x.
+//
+// Column jobs wait for each other (the carry flags): the job at position q waits for position
+// q - 1 or, for a test's column group 0, for its own test's last group at q + NG - 1; each publishes
+// its carry before it waits. The job at q starts when its work-group's job at q - P has finished
+// waiting, i.e. once q - P - 1 (or q - P + NG - 1) has started: with P >= NG (or P at least the
+// list's length) that is an earlier position, so every job starts, provided every work-group of
+// the launch becomes resident. The host keeps each launch within the resident capacity divided
+// among the concurrently running parts; row jobs never wait.
+template <class BODY>
+__device__ __forceinline__ void persist_loop(const int2 *__restrict__ jobs, const unsigned int *lists, int nl,
+                                             BODY body) {
+    const int list = (int)blockIdx.x % nl, k = (int)blockIdx.x / nl, P = (int)gridDim.x / nl;
+    const int off = (int)ldg(lists + kLsOff + list), cnt = (int)ldg(lists + kLsCnt + list);
+    for (int t = k; t < cnt; t += P) {
+        body(jobs[off + t]);
+        __syncthreads();  // this job's last LDS reads before the next job's writes
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_r_p(const SlotDev *__restrict__ slots,
+                                                         const int2 *__restrict__ jobs, int step,
+                                                         const unsigned int *lists, int nl) {
+    persist_loop(jobs, lists, nl, [&](int2 jb) { cols_fused_fast<R>(slots[jb.x], jb.y, step); });
+}
+template <int R1, int R2>
+__global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr_p(const SlotDev *__restrict__ slots,
+                                                          const int2 *__restrict__ jobs, int step,
+                                                          const unsigned int *lists, int nl) {
+    persist_loop(jobs, lists, nl, [&](int2 jb) {
+        const SlotDev &s = slots[jb.x];
+        if (s.R == R1) cols_fused_fast<R1>(s, jb.y, step);
+        else cols_fused_fast<R2>(s, jb.y, step);
+    });
+}
+
 // The column kernels of one slice: kind is one of ColKind, R the (first) column length; nullptr if
 // the slice has no such kernel.
-enum ColKind : int { kColInv, kColFwd, kColFused, kColPairInv, kColPairFwd, kColPairFused };
+enum ColKind : int { kColInv, kColFwd, kColFused, kColPairInv, kColPairFwd, kColPairFused, kColFusedP, kColPairFusedP };
 PassKernel cols_slice_1(int kind, int R);
 PassKernel cols_slice_2(int kind, int R);
 PassKernel cols_slice_3(int kind, int R);
@@ -95,6 +143,7 @@ template <int R, int NEXT> PassKernel col_kernel(int kind) {
     case kColInv: return (PassKernel)k_cols_inv_r<R>;
     case kColFwd: return (PassKernel)k_cols_fwd_r<R>;
     case kColFused: return (PassKernel)k_cols_fused_r<R>;
+    case kColFusedP: return (PassKernel)k_cols_fused_r_p<R>;
     default: break;
     }
     if constexpr (NEXT != 0) {
@@ -102,6 +151,7 @@ template <int R, int NEXT> PassKernel col_kernel(int kind) {
         case kColPairInv: return (PassKernel)k_cols_inv_rr<R, NEXT>;
         case kColPairFwd: return (PassKernel)k_cols_fwd_rr<R, NEXT>;
         case kColPairFused: return (PassKernel)k_cols_fused_rr<R, NEXT>;
+        case kColPairFusedP: return (PassKernel)k_cols_fused_rr_p<R, NEXT>;
         default: break;
         }
     }

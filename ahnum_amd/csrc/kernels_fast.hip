@@ -7,7 +7,14 @@ namespace kite {
 __global__ void __launch_bounds__(128, 3) k_rows_sq256(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
                                                        int mode, int step) {
     (void)mode;
-    rows256_sq(slots, jobs, step);
+    rows256_sq(slots, jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x), step);
+}
+
+// Persistent launch of the same (kernels_fast.h: persist_loop).
+__global__ void __launch_bounds__(128, 3) k_rows_sq256_p(const SlotDev *__restrict__ slots,
+                                                         const int2 *__restrict__ jobs, int step,
+                                                         const unsigned int *lists, int nl) {
+    persist_loop(jobs, lists, nl, [&](int2 jb) { rows256_sq(slots, jb, step); });
 }
 
 namespace {
@@ -69,6 +76,7 @@ PassKernel fast_rows1(int C) {
 }
 
 PassKernel fast_rows_sq(int C) { return C == kRowsRegC ? (PassKernel)k_rows_sq256 : nullptr; }
+PersistKernel fast_rows_sq_p(int C) { return C == kRowsRegC ? (PersistKernel)k_rows_sq256_p : nullptr; }
 
 PassKernel fast_cols_inv(int R) { return col(kColInv, R); }
 PassKernel fast_cols_fwd(int R) { return col(kColFwd, R); }
@@ -76,6 +84,8 @@ PassKernel fast_cols_fused(int R) { return col(kColFused, R); }
 PassKernel fast_cols_pair_inv(int R1) { return fast_cols_next(R1) ? col(kColPairInv, R1) : nullptr; }
 PassKernel fast_cols_pair_fwd(int R1) { return fast_cols_next(R1) ? col(kColPairFwd, R1) : nullptr; }
 PassKernel fast_cols_pair_fused(int R1) { return fast_cols_next(R1) ? col(kColPairFused, R1) : nullptr; }
+PersistKernel fast_cols_fused_p(int R) { return col(kColFusedP, R); }
+PersistKernel fast_cols_pair_fused_p(int R1) { return fast_cols_next(R1) ? col(kColPairFusedP, R1) : nullptr; }
 
 void all_kernels(PassKernel *out, int *count, int cap) {
     int n = 0;
@@ -87,8 +97,9 @@ void all_kernels(PassKernel *out, int *count, int cap) {
         if (fast_rows1(c) && n < cap) out[n++] = fast_rows1(c);
     }
 #define KITE_COLS_ALL_KERNELS(r, next)                                         \
-    for (int kind = kColInv; kind <= kColPairFused; kind++)                    \
-        if (PassKernel k = (kind >= kColPairInv && !next) ? nullptr : col(kind, r)) \
+    for (int kind = kColInv; kind <= kColPairFusedP; kind++)                   \
+        if (PassKernel k = ((kind == kColPairInv || kind == kColPairFwd || kind == kColPairFused || \
+                             kind == kColPairFusedP) && !next) ? nullptr : col(kind, r)) \
             if (n < cap) out[n++] = k;
     KITE_COLS_ALL(KITE_COLS_ALL_KERNELS)
 #undef KITE_COLS_ALL_KERNELS

@@ -394,6 +394,7 @@ SIGNATURES.update({
     "kite_batch_set_graphs": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_set_parts": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_set_xcds": (ctypes.c_int, [c_p, ctypes.c_int]),
+    "kite_batch_set_persistent": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_debug_fail_hip": (None, [ctypes.c_long]),
     "kite_job_bytes": (c_u64, [c_u32, c_u32]),
     "kite_batch_run": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(RunOpts),
@@ -516,6 +517,10 @@ class Batch:
     def graphs(self, on):
         """Replay multi-step square calls from a captured HIP graph (the default) or launch directly."""
         self.lib.kite_batch_set_graphs(self.p, int(bool(on)))
+
+    def persistent(self, on):
+        """Persistent launches of the square-mode passes (kite_batch_set_persistent)."""
+        return self.lib.kite_batch_set_persistent(self.p, int(bool(on)))
 
     def timing(self, every):
         """Time each launch of every `every`-th step of later square calls (0/False: off)."""

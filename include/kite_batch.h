@@ -41,6 +41,14 @@ int kite_batch_set_parts(kite_batch *b, int parts);
 // MI355X, the default): each test's column groups run on one XCD. 1 keeps plain slot order.
 // Returns the value in use.
 int kite_batch_set_xcds(kite_batch *b, int xcds);
+// Persistent launches (off by default): the square-mode row and fused column passes run as
+// persistent kernels where those exist, on at most the device's resident capacity divided among
+// the parts, each work-group running a fixed share of the jobs, instead of one work-group per job.
+// Two batches must not run persistent launches at the same time on one device: the launches
+// would exceed what the device holds at once, and a fused column job would then wait for one
+// that cannot start (its wait is bounded: the test then reports a round-off failure). Returns the
+// setting in use.
+int kite_batch_set_persistent(kite_batch *b, int on);
 // Square calls of more than one step are captured as a HIP graph and replayed while the batch's
 // slots are unchanged (on by default; the launches are the same either way).
 int kite_batch_set_graphs(kite_batch *b, int on);
