@@ -718,7 +718,8 @@ __device__ __forceinline__ void cols_fwd_body(const SlotDev *__restrict__ slots,
 
 // The work-item id through an opaque copy: in a kernel that loops over jobs, everything derived
 // from the id would otherwise be hoisted out of the loop as invariant and kept live across the
-// whole job body (which spilled hundreds of bytes per thread); each job reads its own copy.
+// whole job body (which spilled hundreds of bytes per thread); each job reads its own copy. (The
+// one-job kernels read threadIdx.x directly: the opaque copy costs them about 3%.)
 __device__ __forceinline__ int tid_x() {
     int t = (int)threadIdx.x;
     asm volatile("" : "+v"(t));
@@ -789,12 +790,13 @@ struct NoPre {
 
 // pre() runs once, after the first stage's LDS writes and before its barrier: a kernel stores
 // register-held tables to LDS there, after the first stage's HBM loads have been issued.
-template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, class LD, class ST,
+// OT: the work-item id through tid_x (for kernels that loop over jobs).
+template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, bool OT = false, class LD, class ST,
           class PRE = NoPre>
 __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST &st, PRE pre = PRE()) {
     constexpr int NST = io_nstages(N);
     if constexpr (I < NST) {
-        const int tid = tid_x();
+        const int tid = OT ? tid_x() : (int)threadIdx.x;
         constexpr int P = io_radix(N, I), NB = N / P, TOTAL = NB * HOW;
         constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
         constexpr bool FROM_LD = LDIN && I == 0, TO_ST = STOUT && I == NST - 1;
@@ -843,7 +845,7 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
             }
             if constexpr (I == 0) pre();
             __syncthreads();
-            fft_io<N, NS * P, I + 1, HOW, IS, INV, LDIN, STOUT>(a, tw, ld, st);
+            fft_io<N, NS * P, I + 1, HOW, IS, INV, LDIN, STOUT, OT>(a, tw, ld, st);
         }
     }
 }
@@ -1034,6 +1036,7 @@ __device__ __forceinline__ void xpose_ri(double2 *v, double *X, WR wr, RD rd) {
     __syncthreads();
 }
 
+template <bool OT = false>  // OT: the work-item id through tid_x (a kernel looping over jobs)
 __device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, int2 jb, int step) {
     constexpr int C = kRowsRegC;
     __shared__ double xs[8 * 256];  // one 2 KB block per 16-lane group
@@ -1046,7 +1049,7 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, in
         asm volatile("" ::"s"(count), "s"(R0), "s"(p0), "s"(p1), "s"(p2));
         if (step >= (int)count) return;
     }
-    const int lane = tid_x(), g = lane >> 4, t = lane & 15;
+    const int lane = OT ? tid_x() : (int)threadIdx.x, g = lane >> 4, t = lane & 15;
     const int side = g >> 2, p = jb.y * 4 + (g & 3);
     const int R = s.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
     const bool special = p == 0;  // rows 0 and R/2, each paired with itself
@@ -1062,6 +1065,10 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, in
     __shared__ double2 tws[256];
     static_assert(kRowsSqThreads * 2 == 256, "two twiddles per thread");
     const double2 tw_a = gld(s.tw16 + lane), tw_b = gld(s.tw16 + lane + kRowsSqThreads);
+    // the split's twiddle base (side a's w_H^{p + R t}), fetched with the row: issued after the FFT
+    // its round trip stood between the FFT and the split
+    const bool first = jb.y == 0;  // uniform over the work-group: holds rows 0 and R/2
+    const double2 T0 = (act && !first) ? gld(s.twRows + (size_t)p * C + t) : make_double2(0, 0);
     double2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = act ? gld(xr + 16 * i) : make_double2(0, 0);
@@ -1086,9 +1093,7 @@ __device__ __forceinline__ void rows256_sq(const SlotDev *__restrict__ slots, in
     // its upper half (k2 = 8..15) and combines its lower half, then publishes its lower half and
     // combines its upper half in place
     double2 *Xc = reinterpret_cast<double2 *>(X);
-    const bool first = jb.y == 0;  // uniform over the work-group: holds rows 0 and R/2
     const double2 *blk = (first && special) ? Xc : reinterpret_cast<const double2 *>(xs + (g ^ 4) * 256);
-    const double2 T0 = (act && !first) ? gld(s.twRows + (size_t)p * C + t) : make_double2(0, 0);
     double2 lo[8];
 #pragma unroll
     for (int k2 = 8; k2 < 16; k2++) Xc[(km + 16 * k2) & 127] = v[k2];
@@ -1375,7 +1380,8 @@ template <int R> struct FusedGeom {
     }
 };
 
-template <int R>
+// OT: the work-item id through tid_x (for the persistent kernels, which loop over jobs)
+template <int R, bool OT = false>
 __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int step) {
     extern __shared__ double2 lds[];
     using Gm = FusedGeom<R>;
@@ -1390,7 +1396,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
         asm volatile("" ::"s"(count), "s"(C0), "s"(p0), "s"(p1), "s"(p2), "s"(p3), "s"(p4), "s"(p5));
         if (step >= (int)count) return;
     }
-    const int tid = tid_x();
+    const int tid = OT ? tid_x() : (int)threadIdx.x;
     const int C = s.C, NG = C / G, c0 = g * G;
     const uint32_t seq = s.seq + (uint32_t)step + 1;
     double2 *A = lds, *T = lds + R * GP;
@@ -1462,7 +1468,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
         const double2 *xin = s.x + c0;
         auto ld = [&](int, int, int row, int col) { return cmulc(gld(xin + (size_t)row * C + col), gld(tg + row * G + col)); };
         auto st = [](int, int, int, int, double2) {};
-        fft_io<R, 1, 0, G, GP, true, true, false>(A, T, ld, st, pre);
+        fft_io<R, 1, 0, G, GP, true, true, false, OT>(A, T, ld, st, pre);
     }
     const int b0 = s.b0;
     double lmax = 0;
@@ -1576,7 +1582,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     {
         auto ld = [](int, int, int, int) { return make_double2(0, 0); };
         auto st = [&](int it, int r, int row, int col, double2 v) { gst(xo + row * Co + col, cmul(v, tws[it][r])); };
-        fft_io<R, 1, 0, G, GP, false, false, true>(A, T, ld, st);
+        fft_io<R, 1, 0, G, GP, false, false, true, OT>(A, T, ld, st);
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
     if ((tid & 63) == 0 && lmax > 0)

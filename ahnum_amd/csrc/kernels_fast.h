@@ -115,7 +115,7 @@ template <int R>
 __global__ void __launch_bounds__(kNT, 4) k_cols_fused_r_p(const SlotDev *__restrict__ slots,
                                                          const int2 *__restrict__ jobs, int step,
                                                          const unsigned int *lists, int nl) {
-    persist_loop(jobs, lists, nl, [&](int2 jb) { cols_fused_fast<R>(slots[jb.x], jb.y, step); });
+    persist_loop(jobs, lists, nl, [&](int2 jb) { cols_fused_fast<R, true>(slots[jb.x], jb.y, step); });
 }
 template <int R1, int R2>
 __global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr_p(const SlotDev *__restrict__ slots,
@@ -123,8 +123,8 @@ __global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr_p(const SlotDev *__res
                                                           const unsigned int *lists, int nl) {
     persist_loop(jobs, lists, nl, [&](int2 jb) {
         const SlotDev &s = slots[jb.x];
-        if (s.R == R1) cols_fused_fast<R1>(s, jb.y, step);
-        else cols_fused_fast<R2>(s, jb.y, step);
+        if (s.R == R1) cols_fused_fast<R1, true>(s, jb.y, step);
+        else cols_fused_fast<R2, true>(s, jb.y, step);
     });
 }
 

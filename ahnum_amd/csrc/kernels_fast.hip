@@ -14,7 +14,7 @@ __global__ void __launch_bounds__(128, 3) k_rows_sq256(const SlotDev *__restrict
 __global__ void __launch_bounds__(128, 3) k_rows_sq256_p(const SlotDev *__restrict__ slots,
                                                          const int2 *__restrict__ jobs, int step,
                                                          const unsigned int *lists, int nl) {
-    persist_loop(jobs, lists, nl, [&](int2 jb) { rows256_sq(slots, jb, step); });
+    persist_loop(jobs, lists, nl, [&](int2 jb) { rows256_sq<true>(slots, jb, step); });
 }
 
 namespace {
