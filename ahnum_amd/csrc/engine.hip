@@ -453,6 +453,9 @@ int engine_init(kite_ctx *ctx) {
     s.twRows = sh.twRows;
     s.twCols = sh.twCols;
     s.tw16 = sh.tw16;
+    s.tlo = sh.tlo;
+    s.thi = sh.thi;
+    s.sshift = sh.sshift;
     s.shp = E->shape->dev;
     s.ta = E->ta;
     s.tb = E->tb;
@@ -1079,14 +1082,29 @@ static int batch_refresh(kite_batch *b) {
             std::copy(jobs.begin() + g.off, jobs.begin() + g.off + g.count, cols_orig.begin() + g.off);
         if (b->xcds > 1)
             for (const kite_batch::Group &g : grp) {
+                // each test's jobs (first, count), in order
+                std::vector<std::pair<int, int>> runs;
+                for (int i = g.off; i < g.off + g.count;) {
+                    int j = i;
+                    while (j < g.off + g.count && jobs[j].x == jobs[i].x) j++;
+                    runs.push_back({i, j - i});
+                    i = j;
+                }
+                // Fewer tests than XCDs: each test's jobs are cut into xcds / tests contiguous
+                // chunks, one XCD each, so that neighbouring column groups still share an L2 (with
+                // G = 4 a group's row segments are half cache lines: on different XCDs both halves
+                // would be fetched twice and written back partially). A chunk's first job waits for
+                // the previous chunk's last one, dispatched later; those are at most xcds jobs per
+                // test, far fewer than the device holds at once, so the rest keep running.
+                const int ntests = (int)runs.size();
+                const int per = ntests < b->xcds ? b->xcds / ntests : 1;
                 std::vector<std::vector<int2>> lists(b->xcds);
-                int t = -1, prev = -1;
-                for (int i = g.off; i < g.off + g.count; i++) {
-                    if (jobs[i].x != prev) {
-                        prev = jobs[i].x;
-                        t++;
+                for (int t = 0; t < ntests; t++) {
+                    const int first = runs[t].first, cnt = runs[t].second;
+                    for (int u = 0; u < cnt; u++) {
+                        const int x = per > 1 ? t * per + (int)((int64_t)u * per / cnt) : t % b->xcds;
+                        lists[x].push_back(jobs[first + u]);
                     }
-                    lists[t % b->xcds].push_back(jobs[i]);
                 }
                 std::vector<size_t> pos(b->xcds, 0);
                 int i = g.off;

@@ -81,6 +81,8 @@ struct SlotDev {  // one number being operated on (one test in a batch)
     // dependent load fewer than through shp)
     int R, C, G, NG;
     const double2 *twR, *twC, *twRows, *twCols, *tw16;
+    const double2 *tlo, *thi;   // two-level w_H: w_H^t = thi[t >> sshift] tlo[t mod 2^sshift]
+    int sshift;
     const ShapeDev *shp;        // the whole shape (plans, two-level w_H table) for the generic kernels
     double2 *x;                 // the residue, H complex (after forward column pass)
     const double2 *y_src;       // multiplication operand (stored form), for kModeRowFft

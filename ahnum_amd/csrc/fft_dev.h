@@ -355,6 +355,16 @@ __device__ __forceinline__ void gst4(int32_t *p, int4 v) {
 __device__ __forceinline__ double2 tw_h(const ShapeDev &sh, int t) {
     return cmul(sh.thi[t >> sh.sshift], sh.tlo[t & ((1 << sh.sshift) - 1)]);
 }
+// The same through the slot's copy of the table pointers (global loads). The compile-time kernels
+// use it in place of their H-sized tables (four-step and split twiddles in access order) when those
+// are too large to stay in L2 (R >= 128 columns, C >= 512 rows: megabytes per shape, read from HBM
+// on every pass, more than the residue itself at the longest lengths); the two parts are a few KB.
+// The product of the two correctly rounded factors is within two roundings of w_H^t.
+__device__ __forceinline__ double2 tw_h2(const SlotDev &s, int t) {
+    return cmul(gld(s.thi + (t >> s.sshift)), gld(s.tlo + (t & ((1 << s.sshift) - 1))));
+}
+template <int R> constexpr bool cols_tw2() { return R >= 128; }
+template <int C> constexpr bool rows_tw2() { return C >= 512; }
 
 // ---- real-split pointwise square / multiply ----------------------------------------------------
 // a = U_k, b = U_{H-k}, t = w_H^k; e = a + conj(b), o = -i (a - conj(b)).
@@ -898,8 +908,9 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
     for (int it = 0; it < ITEMS; it++) {
         const int w = it * kNT + (int)threadIdx.x;
         const int i = w / C, kc = w - i * C, p = jb.y * PPW + i;
-        tw_[it] = (split && (NIT % kNT == 0 || w < NIT) && p < npairs) ? gld(s.twRows + (size_t)p * C + kc)
-                                                                          : make_double2(1, 0);
+        const bool need = split && (NIT % kNT == 0 || w < NIT) && p < npairs;
+        if constexpr (rows_tw2<C>()) tw_[it] = need ? tw_h2(s, p + R * kc) : make_double2(1, 0);
+        else tw_[it] = need ? gld(s.twRows + (size_t)p * C + kc) : make_double2(1, 0);
     }
     __syncthreads();
     fft_ip<C, 1, 0, NROW, 1, C, false, false, PD>(A, twC);
@@ -939,7 +950,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
             const int rh = row_of(0, 1);
             pb = C - 1 - kc;
             if (rh >= 0 && pb >= kc) {
-                const double2 t = gld(s.twRows + (size_t)rh * C + kc);
+                const double2 t = rows_tw2<C>() ? tw_h2(s, rh + R * kc) : gld(s.twRows + (size_t)rh * C + kc);
                 double2 a = A[padi<PD>(C + kc)], b = A[padi<PD>(C + pb)];
                 if (mul) split_mul(a, b, t, gld(s.yfreq + (size_t)rh * C + kc), gld(s.yfreq + (size_t)rh * C + pb));
                 else split_sq(a, b, t);
@@ -1466,7 +1477,10 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     // twiddle) straight from HBM; the tables above are published by its first barrier
     {
         const double2 *xin = s.x + c0;
-        auto ld = [&](int, int, int row, int col) { return cmulc(gld(xin + (size_t)row * C + col), gld(tg + row * G + col)); };
+        auto ld = [&](int, int, int row, int col) {
+            if constexpr (cols_tw2<R>()) return cmulc(gld(xin + (size_t)row * C + col), tw_h2(s, (c0 + col) * row));
+            else return cmulc(gld(xin + (size_t)row * C + col), gld(tg + row * G + col));
+        };
         auto st = [](int, int, int, int, double2) {};
         fft_io<R, 1, 0, G, GP, true, true, false, OT>(A, T, ld, st, pre);
     }
@@ -1565,23 +1579,29 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     // bases are laundered so that the inverse side's 64-bit addresses are not kept live (spilled)
     // through the whole kernel for reuse here.
     using LS = LastStage<R, G>;
-    double2 tws[LS::ITER][LS::P];
+    constexpr bool TW2 = cols_tw2<R>();
+    double2 tws[TW2 ? 1 : LS::ITER][LS::P];  // the table's twiddles (two-level: formed at the store)
     const double2 *tgs = tg;
     double2 *xo = s.x + c0;
     int Co = C;
     asm volatile("" : "+v"(tgs), "+v"(xo), "+v"(Co));
+    if constexpr (!TW2) {
 #pragma unroll
-    for (int it = 0; it < LS::ITER; it++) {
-        const int w = it * kNT + tid;
-        if (LS::TOTAL % kNT == 0 || w < LS::TOTAL) {
+        for (int it = 0; it < LS::ITER; it++) {
+            const int w = it * kNT + tid;
+            if (LS::TOTAL % kNT == 0 || w < LS::TOTAL) {
 #pragma unroll
-            for (int r = 0; r < LS::P; r++) tws[it][r] = gld(tgs + (w / G + r * LS::NB) * G + w % G);
+                for (int r = 0; r < LS::P; r++) tws[it][r] = gld(tgs + (w / G + r * LS::NB) * G + w % G);
+            }
         }
     }
     __syncthreads();
     {
         auto ld = [](int, int, int, int) { return make_double2(0, 0); };
-        auto st = [&](int it, int r, int row, int col, double2 v) { gst(xo + row * Co + col, cmul(v, tws[it][r])); };
+        auto st = [&](int it, int r, int row, int col, double2 v) {
+            if constexpr (TW2) gst(xo + row * Co + col, cmul(v, tw_h2(s, (c0 + col) * row)));
+            else gst(xo + row * Co + col, cmul(v, tws[it][r]));
+        };
         fft_io<R, 1, 0, G, GP, false, false, true, OT>(A, T, ld, st);
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));

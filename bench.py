@@ -22,10 +22,11 @@ Beside it, one JSON line carries:
                  test: each pass reads and writes the 8L-byte residue) / average launch duration
   path_roofline  the whole step against HBM (32 L bytes per test) and FP64 (5 L log2(L/2) + 20 L)
   gerbicz_checked  the same tests through kite_batch_run, the production loop (Gerbicz blocks of
-                 1000 with their multiplies, checks every M blocks with limb conversions, round-off
-                 reads), for a bounded time: verified squarings/s as tests/hour
-  serve          a bounded serving run: Poisson arrivals of Fermat and LLR requests (n log-uniform
-                 in [200K, 5M]), p50/p99 completion latency per kind, backlog at the horizon
+                 1000 with their multiplies, a check every 64 blocks on the device, round-off reads)
+                 for a bounded time: its squarings/s as tests/hour, and the verified share
+  serve          a bounded serving run: 200 Poisson arrivals (20/s) of Fermat and LLR requests (n
+                 log-uniform in [200K, 5M]), at most 16 in flight, least work first: p50/p99
+                 completion latency per kind, backlog at the horizon
   cpu_baseline   the oracle's restatement of the reference's portable two-pass backend (c2p),
                  one test per host core, timed on this host
 """
@@ -236,9 +237,11 @@ def serve_only(a):
 
 def gerbicz_leg(tests, secs, blocks):
     """The tests through kite_batch_run for `secs`: Fermat from x_0 = 3^k, Gerbicz blocks of 1000
-    squarings with their multiplies, a check every `blocks` blocks, round-off read every round. The
-    rate counts squarings up to each test's last passed check, from the moment every test had been
-    admitted (their starts, x_0 = 3^k by powering, come first)."""
+    squarings with their multiplies, a check every `blocks` blocks, round-off read every round.
+    Rates start once every test has been admitted (their starts, x_0 = 3^k by powering, come
+    first): tests_per_hour counts every squaring the loop did (those after a test's last check are
+    covered by its next one), verified_tests_per_hour only those up to each test's last passed
+    check (at a horizon of a few checks that undercounts by up to one check interval)."""
     from ahnum_amd import kite
     jobs = [(kite.JOB_PRP, k, n, 0.0) for k, n in tests]
     t0 = time.perf_counter()
@@ -248,9 +251,11 @@ def gerbicz_leg(tests, secs, blocks):
     span = max(min(wall, secs) - admitted, 1e-9)
     verified = sum(r.verified for r in out)
     done = sum(r.iter for r in out)
-    tph = sum(r.verified / span * 3600.0 / n for r, (_, n) in zip(out, tests))
+    tph = sum(r.iter / span * 3600.0 / n for r, (_, n) in zip(out, tests))
+    vtph = sum(r.verified / span * 3600.0 / n for r, (_, n) in zip(out, tests))
     return {
         "tests_per_hour": round(tph, 1),
+        "verified_tests_per_hour": round(vtph, 1),
         "gerbicz_block": 1000,
         "blocks_per_check": blocks,
         "setup_seconds": round(admitted, 2),
@@ -284,9 +289,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gerbicz-secs", type=float, default=30.0,
                     help="length of the Gerbicz-checked leg (0: skip)")
-    ap.add_argument("--gerbicz-blocks", type=int, default=16,
-                    help="blocks of 1000 squarings between checks in the Gerbicz-checked leg")
+    ap.add_argument("--gerbicz-blocks", type=int, default=64,
+                    help="blocks of 1000 squarings between checks in the Gerbicz-checked leg (a check "
+                         "costs one block: 64 keeps it at 1.6%%)")
     ap.add_argument("--serve-secs", type=float, default=30.0, help="horizon of the serving leg (0: skip)")
+    ap.add_argument("--serve-leg-count", type=int, default=200, help="requests of the serving leg")
+    ap.add_argument("--serve-leg-slots", type=int, default=16, help="jobs in flight at most in the serving leg")
     ap.add_argument("--time-every", type=int, default=10,
                     help="bracket the launches of every N-th step of the kernel-timing region with HIP events")
     ap.add_argument("--time-steps", type=int, default=1000, help="steps of the kernel-timing region")
@@ -496,9 +504,9 @@ def main():
         g["vs_value"] = round(g["tests_per_hour"] / out["value"], 4) if out["value"] else None
         out["gerbicz_checked"] = g
     if a.serve_secs > 0 and a.workload == "single_large":
-        out["serve"] = run_serve(count=int(a.serve_secs * a.serve_rate), rate=a.serve_rate, n_lo=a.serve_n_lo,
-                                 n_hi=a.serve_n_hi, horizon=a.serve_secs, hbm_gb=a.serve_hbm_gb, slots=8,
-                                 order_name="least_work", share_name="time")
+        out["serve"] = run_serve(count=a.serve_leg_count, rate=a.serve_rate, n_lo=a.serve_n_lo,
+                                 n_hi=a.serve_n_hi, horizon=a.serve_secs, hbm_gb=a.serve_hbm_gb,
+                                 slots=a.serve_leg_slots, order_name="least_work", share_name="time")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tests[:16], a.cpu_secs)
     print(json.dumps(out), flush=True)
