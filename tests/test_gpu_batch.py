@@ -135,15 +135,16 @@ def test_mixed_batch_all_256_vs_gmp_subset(golden):
     (tests/golden/res64_batch_subset.csv; GMP takes over 40 min per test at these sizes) match."""
     rows = golden("res64_batch_subset.csv")
     tests = workloads.batch_mixed()
-    assert [(int(k), int(n)) for k, n, _, _ in rows] == tests[: len(rows)]
-    say("mixed batch of", len(tests), "tests")
+    assert {(int(k), int(n)) for k, n, _, _ in rows} <= set(tests)  # a subset of the batch's tests
+    say("mixed batch of", len(tests), "tests;", len(rows), "with GMP RES64s")
     rc, out = kite.batch_prp(tests, block=1000)
     assert rc == 0
     assert all(st == 0 and r.maxerr < 0.4 for st, r in out)
     assert len({r.fft_len for _, r in out}) >= 5  # mixed lengths advance together
     say("gerbicz errors", sum(r.gerbicz_errors for _, r in out), "retries", sum(r.roundoff_retries for _, r in out))
-    for (k, n, base, want), (st, r) in zip(rows, out):
-        assert "%016X" % r.res64 == want, (k, n)
+    res = {t: "%016X" % r.res64 for t, (st, r) in zip(tests, out)}
+    for k, n, base, want in rows:
+        assert res[(int(k), int(n))] == want, (k, n)
 
 
 def test_batch_llr_matches_prst(golden):
