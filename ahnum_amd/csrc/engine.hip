@@ -302,6 +302,7 @@ struct Engine {
     int nrowjobs1 = 0;
     size_t lds_rows1 = 0;
     PassKernel kcfused = nullptr;   // square-mode fused column pass, if any
+    int nt_fused = kNT;             // its threads per work-group
     unsigned long long *flags = nullptr;  // its per-chunk carry flags
     uint32_t seq = 0;                     // its launches so far
     SlotDev *d_slot = nullptr;            // the slot table of this context's own launches (one entry)
@@ -487,6 +488,7 @@ int engine_init(kite_ctx *ctx) {
         const bool fused_ok = ff && ctx->b0 >= (uint32_t)kFusedMinBits;
         E->kcfused = fused_ok ? fast_cols_fused(sh.R) : nullptr;
         E->lds_fused = fused_ok ? fast_cols_fused_lds(sh.R) : 0;
+        E->nt_fused = fused_ok ? fast_cols_fused_threads(sh.R) : kNT;
         E->name[0] = E->krows_sq ? "k_rows_sq" + std::to_string(sh.C)
                      : fr       ? "k_rows_c<" + std::to_string(sh.C) + ">"
                                 : std::string("k_rows_g");
@@ -604,7 +606,8 @@ static bool launch_square(Engine *E, int step, const SlotDev *tbl = nullptr) {
         ok = launch_rows(E, kModeSquare, step, tbl);
     if (!ok) return false;
     if (E->kcfused)
-        return chk(E, launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, tbl, nullptr, kModeSquare, step),
+        return chk(E, launch_pass(E->kcfused, E->ncoljobs, E->lds_fused, E->stream, tbl, nullptr, kModeSquare, step,
+                                  E->nt_fused),
                    "fused column pass");
     return launch_cols(E, kModeSquare, step, tbl);
 }
@@ -864,6 +867,7 @@ struct kite_batch {
         kite::PassKernel kf = nullptr;  // columns: square-mode fused pass, if any
         int stat_f = -1;
         size_t lds_f = 0;
+        int nt_f = kite::kNT;  // its threads per work-group
         int part = 0;  // the stream the group's launches go to
         // the persistent variant of the square-mode pass (rows: k2, columns: kf), if any; its job
         // lists (index into d_lists, the list count, the longest list) and the work-groups each list
@@ -1058,6 +1062,7 @@ static int batch_refresh(kite_batch *b) {
                 g.stat[0] = stat_of(E0->name[1], 1);
                 g.stat[1] = stat_of(E0->name[2], 2);
                 g.kf = E0->kcfused;
+                g.nt_f = E0->nt_fused;
                 if (g.kf) g.stat_f = stat_of("k_cols_fused_r<" + std::to_string(E0->shape->ch.R) + ">", 1);
                 g.kp = g.kf ? fast_cols_fused_p(E0->shape->ch.R) : nullptr;
                 if (g.kp) g.stat_p = stat_of("k_cols_fused_r_p<" + std::to_string(E0->shape->ch.R) + ">", 1);
@@ -1133,6 +1138,7 @@ static int batch_refresh(kite_batch *b) {
                 g.stat[0] = stat_of("k_cols_inv_rr" + rr, 1);
                 g.stat[1] = stat_of("k_cols_fwd_rr" + rr, 2);
                 g.kf = (grp[i].kf && grp[i + 1].kf) ? fast_cols_pair_fused(r1) : nullptr;
+                g.nt_f = fast_cols_pair_fused_threads(r1);
                 g.stat_f = g.kf ? stat_of("k_cols_fused_rr" + rr, 1) : -1;
                 g.kp = g.kf ? fast_cols_pair_fused_p(r1) : nullptr;
                 g.stat_p = g.kp ? stat_of("k_cols_fused_rr_p" + rr, 1) : -1;
@@ -1300,7 +1306,7 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
         if (fused && pass == 2) continue;
         const PassKernel fn = fused ? g.kf : (pass == 2 || rsq) ? g.k2 : g.k1;
         const size_t lds = rsq ? 0 : fused ? g.lds_f : g.lds;
-        const int threads = rsq ? kRowsSqThreads : kNT;
+        const int threads = rsq ? kRowsSqThreads : fused ? g.nt_f : kNT;
         const int pgrid = b->persist && g.kp && (rsq || fused) ? persist_grid(b, g, threads, lds) : 0;
         const bool persist = pgrid > 0;
         auto launch = [&]() -> hipError_t {

@@ -787,12 +787,31 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
 // butterfly input/output, (row, col) the element.
 // Radix plan of fft_io: two stages (N/8, then 8) when N/8 is a length of Dft, which halves the
 // LDS passes and barriers of the column lengths up to 128; else the plan of the other kernels.
-__host__ __device__ constexpr bool io_two(int n) {
-    return n % 8 == 0 && (n / 8 == 2 || n / 8 == 3 || n / 8 == 4 || n / 8 == 5 || n / 8 == 6 || n / 8 == 7 ||
-                          n / 8 == 8 || n / 8 == 10 || n / 8 == 12 || n / 8 == 14 || n / 8 == 16);
+// For the column lengths up to 128 whose R/8 is no DFT length here, the most balanced split into
+// two DFT lengths (100 = 10 x 10, 36 = 6 x 6, 120 = 12 x 10), else the plan of the other kernels.
+__host__ __device__ constexpr bool dft_len(int n) {
+    return n == 2 || n == 3 || n == 4 || n == 5 || n == 6 || n == 7 || n == 8 || n == 10 || n == 12 || n == 14 ||
+           n == 16;
 }
+__host__ __device__ constexpr int io_first(int n) {
+    if (n % 8 == 0 && dft_len(n / 8)) return n / 8;
+    int best = 0, bd = 0;
+    if (n <= 128)
+        for (int a = 2; a <= 16; a++)
+            if (n % a == 0 && dft_len(a) && dft_len(n / a)) {
+                const int d = a > n / a ? a - n / a : n / a - a;
+                if (!best || d < bd) {
+                    best = a;
+                    bd = d;
+                }
+            }
+    return best;
+}
+__host__ __device__ constexpr bool io_two(int n) { return io_first(n) != 0; }
 __host__ __device__ constexpr int io_nstages(int n) { return io_two(n) ? 2 : ct_nstages(n); }
-__host__ __device__ constexpr int io_radix(int n, int idx) { return io_two(n) ? (idx == 0 ? n / 8 : 8) : ct_radix(n, idx); }
+__host__ __device__ constexpr int io_radix(int n, int idx) {
+    return io_two(n) ? (idx == 0 ? io_first(n) : n / io_first(n)) : ct_radix(n, idx);
+}
 
 struct NoPre {
     __device__ void operator()() const {}
@@ -800,22 +819,29 @@ struct NoPre {
 
 // pre() runs once, after the first stage's LDS writes and before its barrier: a kernel stores
 // register-held tables to LDS there, after the first stage's HBM loads have been issued.
-// OT: the work-item id through tid_x (for kernels that loop over jobs).
-template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, bool OT = false, class LD, class ST,
-          class PRE = NoPre>
+// OT: the work-item id through tid_x (for kernels that loop over jobs); NT: threads per work-group.
+// A compile-time bound the compiler may not use to drop a phase's thread test (the test keeps the
+// phase's code predicated, which at the fused kernel's register limit schedules without spills).
+__device__ __forceinline__ int opaque_bound(int n) {
+    asm volatile("" : "+s"(n));
+    return n;
+}
+
+template <int N, int NS, int I, int HOW, int IS, bool INV, bool LDIN, bool STOUT, bool OT = false, int NT = kNT,
+          class LD, class ST, class PRE = NoPre>
 __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST &st, PRE pre = PRE()) {
     constexpr int NST = io_nstages(N);
     if constexpr (I < NST) {
         const int tid = OT ? tid_x() : (int)threadIdx.x;
         constexpr int P = io_radix(N, I), NB = N / P, TOTAL = NB * HOW;
-        constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
+        constexpr int ITER = (TOTAL + NT - 1) / NT, TS = N / (NS * P);
         constexpr bool FROM_LD = LDIN && I == 0, TO_ST = STOUT && I == NST - 1;
         double2 v[ITER][P];
         int orow[ITER], seqs[ITER];
 #pragma unroll
         for (int it = 0; it < ITER; it++) {
-            const int w = it * kNT + tid;
-            if (TOTAL % kNT == 0 || w < TOTAL) {
+            const int w = it * NT + tid;
+            if (w < opaque_bound(TOTAL)) {
                 const int seq = w % HOW, j = w / HOW, jm = j % NS;
 #pragma unroll
                 for (int r = 0; r < P; r++) {
@@ -837,8 +863,8 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
         if constexpr (TO_ST) {
 #pragma unroll
             for (int it = 0; it < ITER; it++) {
-                const int w = it * kNT + tid;
-                if (TOTAL % kNT == 0 || w < TOTAL) {
+                const int w = it * NT + tid;
+                if (w < opaque_bound(TOTAL)) {
 #pragma unroll
                     for (int r = 0; r < P; r++) st(it, r, orow[it] + r * NS, seqs[it], v[it][r]);
                 }
@@ -847,23 +873,23 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
             if constexpr (!FROM_LD) __syncthreads();  // the stage is in place: reads before writes
 #pragma unroll
             for (int it = 0; it < ITER; it++) {
-                const int w = it * kNT + tid;
-                if (TOTAL % kNT == 0 || w < TOTAL) {
+                const int w = it * NT + tid;
+                if (w < opaque_bound(TOTAL)) {
 #pragma unroll
                     for (int r = 0; r < P; r++) a[(orow[it] + r * NS) * IS + seqs[it]] = v[it][r];
                 }
             }
             if constexpr (I == 0) pre();
             __syncthreads();
-            fft_io<N, NS * P, I + 1, HOW, IS, INV, LDIN, STOUT, OT>(a, tw, ld, st);
+            fft_io<N, NS * P, I + 1, HOW, IS, INV, LDIN, STOUT, OT, NT>(a, tw, ld, st);
         }
     }
 }
 
 // The last stage's thread mapping of fft_io (for prefetching what its outputs need).
-template <int N, int HOW> struct LastStage {
+template <int N, int HOW, int NT = kNT> struct LastStage {
     static constexpr int P = io_radix(N, io_nstages(N) - 1), NB = N / P, TOTAL = NB * HOW;
-    static constexpr int ITER = (TOTAL + kNT - 1) / kNT;
+    static constexpr int ITER = (TOTAL + NT - 1) / NT;
 };
 
 // Rows: 2*PPW rows of C in LDS (row slot 2i+h = side h of local pair i), twiddles w_C after them.
@@ -1379,9 +1405,27 @@ __device__ __forceinline__ double bal_e(double x, int b, double &d) {
     return q;
 }
 
-template <int R> struct FusedGeom {
+// Threads per work-group of the fused column pass at column length R: the most any of its phases
+// uses (the FFT stages, one thread per carry sub-chunk), in whole waves, at most kNT. At R = 96
+// (G = 16) every phase needs at most 192 threads: a fourth wave would hold registers and idle, and
+// three-wave work-groups fit five to a CU where four-wave ones fit four.
+template <int N, int HOW, int I = 0> constexpr int io_max_total() {
+    if constexpr (I >= io_nstages(N)) return 0;
+    else {
+        constexpr int t = N / io_radix(N, I) * HOW, r = io_max_total<N, HOW, I + 1>();
+        return t > r ? t : r;
+    }
+}
+template <int R> constexpr int fused_nt() {
+    constexpr int G = g_for_R(R), SUB = G < kSub ? G : kSub, NSC = R * (G / SUB);
+    constexpr int m = io_max_total<R, G>() > NSC ? io_max_total<R, G>() : NSC;
+    constexpr int w = (m + 63) / 64 * 64;
+    return w < kNT ? w : kNT;
+}
+
+template <int R, int NT = kNT> struct FusedGeom {
     static constexpr int G = g_for_R(R), GP = G + 1, SUB = G < kSub ? G : kSub, NSUB = G / SUB;
-    static constexpr int NE = R * G, NSC = R * NSUB, ITSC = (NSC + kNT - 1) / kNT, ITE = (NE + kNT - 1) / kNT;
+    static constexpr int NE = R * G, NSC = R * NSUB, ITSC = (NSC + NT - 1) / NT, ITE = (NE + NT - 1) / NT;
     // LDS: R x G elements with row stride GP (the padding spreads the sub-chunk reads over the
     // banks), the column twiddles, one carry per sub-chunk, the group's weight factors (inverse and
     // forward: 2G column and R row values each), a 64-digit scratch for the fold. (The digit codes
@@ -1391,11 +1435,12 @@ template <int R> struct FusedGeom {
     }
 };
 
-// OT: the work-item id through tid_x (for the persistent kernels, which loop over jobs)
-template <int R, bool OT = false>
+// OT: the work-item id through tid_x (for the persistent kernels, which loop over jobs); NT: threads
+// per work-group (any multiple of 64: phases needing more threads iterate)
+template <int R, bool OT = false, int NT = kNT>
 __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int step) {
     extern __shared__ double2 lds[];
-    using Gm = FusedGeom<R>;
+    using Gm = FusedGeom<R, NT>;
     constexpr int G = Gm::G, GP = Gm::GP, SUB = Gm::SUB, NSUB = Gm::NSUB, NE = Gm::NE, NSC = Gm::NSC;
     constexpr int ITSC = Gm::ITSC, ITE = Gm::ITE;
     {
@@ -1418,7 +1463,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     // The group's tables (column twiddles, weight factors) are loaded into registers here and
     // stored to LDS by pre() below, once the first FFT stage has issued its HBM loads: their round
     // trips overlap instead of preceding the residue's.
-    constexpr int NTR = (R + kNT - 1) / kNT, NTB = (2 * G + kNT - 1) / kNT;
+    constexpr int NTR = (R + NT - 1) / NT, NTB = (2 * G + NT - 1) / NT;
     double2 rT[NTR];
     double rWA[NTR], rWA2[NTR], rWB[NTB], rWB2[NTB];
     // The digit codes of the thread's own sub-chunks (SUB bytes each: row r, columns q SUB ..),
@@ -1427,9 +1472,9 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     uint64_t cdw[ITSC];
 #pragma unroll
     for (int it = 0; it < ITSC; it++) {
-        const int w = it * kNT + tid;
+        const int w = it * NT + tid;
         cdw[it] = 0;
-        if (NSC % kNT != 0 && w >= NSC) continue;
+        if (w >= opaque_bound(NSC)) continue;
         const int q = w / R, r = w - q * R;
         const uint8_t *cp = s.dcode + ((size_t)g * R + r) * G + q * SUB;
         if constexpr (SUB == 8) cdw[it] = ldg(reinterpret_cast<const unsigned long long *>(cp));
@@ -1437,8 +1482,8 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     }
 #pragma unroll
     for (int q = 0; q < NTR; q++) {
-        const int i = q * kNT + tid;
-        if (R % kNT == 0 || i < R) {
+        const int i = q * NT + tid;
+        if (R % NT == 0 || i < R) {
             rT[q] = gld(s.twR + i);
             rWA[q] = ldg(s.ta + R + i);
             rWA2[q] = ldg(s.ta + i);
@@ -1446,8 +1491,8 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     }
 #pragma unroll
     for (int q = 0; q < NTB; q++) {
-        const int i = q * kNT + tid;
-        if ((2 * G) % kNT == 0 || i < 2 * G) {
+        const int i = q * NT + tid;
+        if ((2 * G) % NT == 0 || i < 2 * G) {
             rWB[q] = ldg(s.tb + 2 * C + 2 * c0 + i);
             rWB2[q] = ldg(s.tb + 2 * c0 + i);
         }
@@ -1456,8 +1501,8 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     auto pre = [&]() {
 #pragma unroll
         for (int q = 0; q < NTR; q++) {
-            const int i = q * kNT + tid;
-            if (R % kNT == 0 || i < R) {
+            const int i = q * NT + tid;
+            if (R % NT == 0 || i < R) {
                 T[i] = rT[q];
                 WA[i] = rWA[q] * kscale;
                 WA[R + i] = rWA2[q];
@@ -1465,8 +1510,8 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
         }
 #pragma unroll
         for (int q = 0; q < NTB; q++) {
-            const int i = q * kNT + tid;
-            if ((2 * G) % kNT == 0 || i < 2 * G) {
+            const int i = q * NT + tid;
+            if ((2 * G) % NT == 0 || i < 2 * G) {
                 WB[i] = rWB[q];
                 WB[2 * G + i] = rWB2[q];
             }
@@ -1482,7 +1527,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
             else return cmulc(gld(xin + (size_t)row * C + col), gld(tg + row * G + col));
         };
         auto st = [](int, int, int, int, double2) {};
-        fft_io<R, 1, 0, G, GP, true, true, false, OT>(A, T, ld, st, pre);
+        fft_io<R, 1, 0, G, GP, true, true, false, OT, NT>(A, T, ld, st, pre);
     }
     const int b0 = s.b0;
     double lmax = 0;
@@ -1490,8 +1535,8 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     uint32_t bg[ITSC];  // bit i: digit i of the sub-chunk has b0+1 bits
 #pragma unroll
     for (int it = 0; it < ITSC; it++) {
-        const int w = it * kNT + tid;
-        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int w = it * NT + tid;
+        if (w >= opaque_bound(NSC)) continue;
         // sub-chunk (r, q) is w = q R + r: the lanes of a wave take consecutive rows, so their
         // reads of A (row stride G + 1) fall in distinct LDS banks
         const int q = w / R, r = w - q * R;
@@ -1532,7 +1577,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     __syncthreads();
     // the top fold, here where few registers are live (its 128-bit arithmetic inside the loop above
     // made every thread spill); the last sub-chunk's carry goes nowhere inside the work-group
-    if (g == NG - 1 && tid == (NSC - 1) % kNT) {
+    if (g == NG - 1 && tid == (NSC - 1) % NT) {
         constexpr int it = ITSC - 1;
         const int64_t co = fold_top_dev(s, FD, SC[NSC - 1]);
         SC[NSC - 1] = co;
@@ -1545,8 +1590,8 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     // flag); the last digit keeps the rest. Then weight into A for the forward column FFT.
 #pragma unroll
     for (int it = 0; it < ITSC; it++) {
-        const int w = it * kNT + tid;
-        if (NSC % kNT != 0 && w >= NSC) continue;
+        const int w = it * NT + tid;
+        if (w >= opaque_bound(NSC)) continue;
         const int q = w / R, r = w - q * R;
         int64_t cy;
         if (q > 0) {
@@ -1578,7 +1623,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     // to HBM. The twiddles of the thread's last-stage outputs are fetched now, during the FFT; the
     // bases are laundered so that the inverse side's 64-bit addresses are not kept live (spilled)
     // through the whole kernel for reuse here.
-    using LS = LastStage<R, G>;
+    using LS = LastStage<R, G, NT>;
     constexpr bool TW2 = cols_tw2<R>();
     double2 tws[TW2 ? 1 : LS::ITER][LS::P];  // the table's twiddles (two-level: formed at the store)
     const double2 *tgs = tg;
@@ -1588,8 +1633,8 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
     if constexpr (!TW2) {
 #pragma unroll
         for (int it = 0; it < LS::ITER; it++) {
-            const int w = it * kNT + tid;
-            if (LS::TOTAL % kNT == 0 || w < LS::TOTAL) {
+            const int w = it * NT + tid;
+            if (w < opaque_bound(LS::TOTAL)) {
 #pragma unroll
                 for (int r = 0; r < LS::P; r++) tws[it][r] = gld(tgs + (w / G + r * LS::NB) * G + w % G);
             }
@@ -1602,7 +1647,7 @@ __device__ __forceinline__ void cols_fused_fast(const SlotDev &s, int g, int ste
             if constexpr (TW2) gst(xo + row * Co + col, cmul(v, tw_h2(s, (c0 + col) * row)));
             else gst(xo + row * Co + col, cmul(v, tws[it][r]));
         };
-        fft_io<R, 1, 0, G, GP, false, false, true, OT>(A, T, ld, st);
+        fft_io<R, 1, 0, G, GP, false, false, true, OT, NT>(A, T, ld, st);
     }
     for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
     if ((tid & 63) == 0 && lmax > 0)

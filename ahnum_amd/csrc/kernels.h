@@ -37,6 +37,10 @@ PassKernel fast_cols_pair_fwd(int R);
 // square mode: both column passes in one kernel, carries between work-groups through flags
 PassKernel fast_cols_fused(int R);
 size_t fast_cols_fused_lds(int R);  // LDS bytes of the fused column kernel
+// threads per work-group of the fused column kernels of length R (and of the pair kernels whose
+// first length is R)
+int fast_cols_fused_threads(int R);
+int fast_cols_pair_fused_threads(int R1);
 PassKernel fast_cols_pair_fused(int R);
 // persistent variants (nullptr: none): the register row kernel, the fused column kernels
 PersistKernel fast_rows_sq_p(int C);

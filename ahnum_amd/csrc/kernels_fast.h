@@ -45,12 +45,16 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *__restrict__ 
                                                   int mode, int step) {
     cols_fwd_fast<R>(slots, jobs, mode, step);
 }
+// The fused kernels run fused_nt<R>() threads per work-group, the pair kernels the larger of their
+// two lengths' counts (fewer would make the longer length iterate, which spilled registers);
+// fast_cols_fused_threads and fast_cols_pair_fused_threads tell the host.
+template <int R1, int R2> constexpr int fused_nt2() { return fused_nt<R1>() > fused_nt<R2>() ? fused_nt<R1>() : fused_nt<R2>(); }
 template <int R>
-__global__ void __launch_bounds__(kNT, 4) k_cols_fused_r(const SlotDev *__restrict__ slots,
-                                                       const int2 *__restrict__ jobs, int mode, int step) {
+__global__ void __launch_bounds__(fused_nt<R>(), 4) k_cols_fused_r(const SlotDev *__restrict__ slots,
+                                                                 const int2 *__restrict__ jobs, int mode, int step) {
     (void)mode;
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
-    cols_fused_fast<R>(slots[jb.x], jb.y, step);
+    cols_fused_fast<R, false, fused_nt<R>()>(slots[jb.x], jb.y, step);
 }
 
 // Two adjacent column lengths in one launch: tests of both shapes advance together (each work-
@@ -71,13 +75,13 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_rr(const SlotDev *__restrict__
     else cols_fwd_fast<R2>(slots, jobs, mode, step);
 }
 template <int R1, int R2>
-__global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr(const SlotDev *__restrict__ slots,
-                                                        const int2 *__restrict__ jobs, int mode, int step) {
+__global__ void __launch_bounds__((fused_nt2<R1, R2>()), 4) k_cols_fused_rr(const SlotDev *__restrict__ slots,
+                                                                   const int2 *__restrict__ jobs, int mode, int step) {
     (void)mode;
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     const SlotDev &s = slots[jb.x];
-    if (s.R == R1) cols_fused_fast<R1>(s, jb.y, step);
-    else cols_fused_fast<R2>(s, jb.y, step);
+    if (s.R == R1) cols_fused_fast<R1, false, fused_nt2<R1, R2>()>(s, jb.y, step);
+    else cols_fused_fast<R2, false, fused_nt2<R1, R2>()>(s, jb.y, step);
 }
 
 // ---- persistent launches ----------------------------------------------------------------------
@@ -112,19 +116,20 @@ __device__ __forceinline__ void persist_loop(const int2 *__restrict__ jobs, cons
 }
 
 template <int R>
-__global__ void __launch_bounds__(kNT, 4) k_cols_fused_r_p(const SlotDev *__restrict__ slots,
-                                                         const int2 *__restrict__ jobs, int step,
-                                                         const unsigned int *lists, int nl) {
-    persist_loop(jobs, lists, nl, [&](int2 jb) { cols_fused_fast<R, true>(slots[jb.x], jb.y, step); });
+__global__ void __launch_bounds__(fused_nt<R>(), 4) k_cols_fused_r_p(const SlotDev *__restrict__ slots,
+                                                                   const int2 *__restrict__ jobs, int step,
+                                                                   const unsigned int *lists, int nl) {
+    persist_loop(jobs, lists, nl,
+                 [&](int2 jb) { cols_fused_fast<R, true, fused_nt<R>()>(slots[jb.x], jb.y, step); });
 }
 template <int R1, int R2>
-__global__ void __launch_bounds__(kNT, 4) k_cols_fused_rr_p(const SlotDev *__restrict__ slots,
-                                                          const int2 *__restrict__ jobs, int step,
-                                                          const unsigned int *lists, int nl) {
+__global__ void __launch_bounds__((fused_nt2<R1, R2>()), 4) k_cols_fused_rr_p(const SlotDev *__restrict__ slots,
+                                                                     const int2 *__restrict__ jobs, int step,
+                                                                     const unsigned int *lists, int nl) {
     persist_loop(jobs, lists, nl, [&](int2 jb) {
         const SlotDev &s = slots[jb.x];
-        if (s.R == R1) cols_fused_fast<R1, true>(s, jb.y, step);
-        else cols_fused_fast<R2, true>(s, jb.y, step);
+        if (s.R == R1) cols_fused_fast<R1, true, fused_nt2<R1, R2>()>(s, jb.y, step);
+        else cols_fused_fast<R2, true, fused_nt2<R1, R2>()>(s, jb.y, step);
     });
 }
 

@@ -44,6 +44,24 @@ size_t fast_cols_lds(int R) {
     }
 }
 
+int fast_cols_fused_threads(int R) {
+    switch (R) {
+#define KITE_FUSED_NT(r, next) case r: return fused_nt<r>();
+        KITE_COLS_ALL(KITE_FUSED_NT)
+#undef KITE_FUSED_NT
+    default: return kNT;
+    }
+}
+
+int fast_cols_pair_fused_threads(int R1) {
+    switch (R1) {
+#define KITE_PAIR_NT(r, next) case r: return next ? fused_nt2<r, next ? next : r>() : fused_nt<r>();
+        KITE_COLS_ALL(KITE_PAIR_NT)
+#undef KITE_PAIR_NT
+    default: return kNT;
+    }
+}
+
 int fast_cols_next(int R) {
     switch (R) {
 #define KITE_COLS_NEXT(r, next) case r: return next;
