@@ -880,6 +880,10 @@ struct kite_batch {
     // concurrently, so their kernels overlap in different phases (loads, compute, stores).
     static constexpr int kMaxParts = 4;
     int nparts = 2;
+    // kite_batch_set_shape_parts: the most common shape's slots go to the nparts parts, each other
+    // shape's to a stream of its own while streams are left; nused = streams in use (refresh)
+    int shape_parts = 0;
+    int nused = 2;
     hipStream_t ps[kMaxParts] = {};  // ps[0] == stream
     hipEvent_t evfork = nullptr, evtable = nullptr, evjoin[kMaxParts] = {};
     std::vector<int> slot_part;
@@ -907,6 +911,7 @@ struct kite_batch {
     // timing: one event pair per launch of the last timed call, and what each pair measured
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_stat;
+    std::vector<double> ev_bytes;  // algorithmic bytes of each timed launch
     double ms[3] = {0, 0, 0};
     uint64_t launches[3] = {0, 0, 0};
     struct KStat {
@@ -1010,11 +1015,41 @@ static int batch_refresh(kite_batch *b) {
         b->kstats.push_back(ks);
         return (int)b->kstats.size() - 1;
     };
-    // parts: active slots dealt round-robin (by slot order) to the parts
+    // parts: active slots dealt round-robin (by slot order) to the parts; with shape_parts, only the
+    // most common shape's, each other shape taking a part of its own while streams are left (its
+    // short launches then run beside the main parts' rather than after them in one stream, and
+    // lengths that would share a pair kernel each keep their own kernel)
     b->slot_part.assign(cap, 0);
+    b->nused = std::min<int>(b->nparts, std::max<int>(1, (int)order.size()));
     {
+        std::vector<const void *> own(cap, nullptr);  // a shape with a part of its own -> that part
+        std::vector<int> own_part;
+        std::vector<const void *> own_key;
+        int avail = 0;  // streams that exist (ps[0] always)
+        while (avail < kite_batch::kMaxParts && b->ps[avail]) avail++;
+        if (b->shape_parts && b->nparts < avail) {
+            std::vector<std::pair<int, const void *>> cnt;  // (slots, shape)
+            for (int s : order) {
+                const void *key = b->slots[s].ctx->eng->shape;
+                auto it = std::find_if(cnt.begin(), cnt.end(), [&](const std::pair<int, const void *> &c) { return c.second == key; });
+                if (it == cnt.end()) cnt.push_back({1, key});
+                else it->first++;
+            }
+            std::stable_sort(cnt.begin(), cnt.end(), [](const std::pair<int, const void *> &x, const std::pair<int, const void *> &y) {
+                return x.first > y.first;
+            });
+            for (size_t i = 1; i < cnt.size() && b->nparts + (int)own_key.size() < avail; i++) {
+                own_key.push_back(cnt[i].second);
+                own_part.push_back(b->nparts + (int)own_key.size() - 1);
+            }
+            if (!own_key.empty()) b->nused = b->nparts + (int)own_key.size();
+        }
         int i = 0;
-        for (int s : order) b->slot_part[s] = (i++) % b->nparts;
+        for (int s : order) {
+            const void *key = b->slots[s].ctx->eng->shape;
+            const auto it = std::find(own_key.begin(), own_key.end(), key);
+            b->slot_part[s] = it != own_key.end() ? own_part[it - own_key.begin()] : (i++) % b->nparts;
+        }
     }
     auto build = [&](bool isrows, int part) {
         std::vector<int2> &jobs = isrows ? rows : cols;
@@ -1150,7 +1185,7 @@ static int batch_refresh(kite_batch *b) {
         }
         out.insert(out.end(), merged.begin(), merged.end());
     };
-    for (int part = 0; part < b->nparts; part++) {
+    for (int part = 0; part < b->nused; part++) {
         build(true, part);
         build(false, part);
     }
@@ -1261,9 +1296,9 @@ static int batch_upload(kite_batch *b, const int32_t *xi, const int32_t *yi, con
     BATCH_OK(b, hipMemcpyAsync(b->d_slots, b->h_slots, b->slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice,
                                b->stream));
     // the other parts' launches read the table too: they wait for the copy
-    if (b->nparts > 1) {
+    if (b->nused > 1) {
         BATCH_OK(b, hipEventRecord(b->evtable, b->stream));
-        for (int p = 1; p < b->nparts; p++) BATCH_OK(b, hipStreamWaitEvent(b->ps[p], b->evtable, 0));
+        for (int p = 1; p < b->nused; p++) BATCH_OK(b, hipStreamWaitEvent(b->ps[p], b->evtable, 0));
     }
     return 0;
 }
@@ -1290,7 +1325,7 @@ static int persist_grid(kite_batch *b, const kite_batch::Group &g, int threads, 
             cache.push_back({{g.kp, lds}, cap});
         }
     }
-    const int P = std::min(cap / std::max(1, b->nparts) / g.nl, g.maxcnt);
+    const int P = std::min(cap / std::max(1, b->nused) / g.nl, g.maxcnt);
     return P >= g.pmin && P > 0 ? P * g.nl : 0;
 }
 
@@ -1309,13 +1344,16 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
         const int threads = rsq ? kRowsSqThreads : fused ? g.nt_f : kNT;
         const int pgrid = b->persist && g.kp && (rsq || fused) ? persist_grid(b, g, threads, lds) : 0;
         const bool persist = pgrid > 0;
+        // a call with timing on runs every launch on the first stream, so that each timed launch
+        // has the device to itself (every call starts after all streams' earlier work)
+        const hipStream_t st = b->timing ? b->ps[0] : b->ps[g.part];
         auto launch = [&]() -> hipError_t {
-            if (!persist) return launch_pass(fn, g.count, lds, b->ps[g.part], b->d_slots, jobs + g.off, mode, step, threads);
+            if (!persist) return launch_pass(fn, g.count, lds, st, b->d_slots, jobs + g.off, mode, step, threads);
             const int2 *pj = (pass == 0 ? b->d_rows : b->d_cols_p) + g.off;
             const unsigned int *ls = b->d_lists + (size_t)g.ls * kPersistListWords;
             int nl = g.nl;
             void *args[] = {(void *)&b->d_slots, (void *)&pj, (void *)&step, (void *)&ls, (void *)&nl};
-            return hipLaunchKernel(g.kp, dim3(pgrid), dim3(threads), args, lds, b->ps[g.part]);
+            return hipLaunchKernel(g.kp, dim3(pgrid), dim3(threads), args, lds, st);
         };
         hipError_t e;
         if (timed) {
@@ -1326,9 +1364,12 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
                 hipEventCreate(&b->ev[ei + 1]);
             }
             b->ev_stat.push_back(persist ? g.stat_p : fused ? g.stat_f : g.stat[pass == 2 ? 1 : 0]);
-            hipEventRecord(b->ev[ei], b->ps[g.part]);
+            // algorithmic bytes: each test's residue (8 L) read and written by the row pass and the
+            // fused column pass, read or written by each split column pass
+            b->ev_bytes.push_back((pass == 0 || fused ? 16.0 : 8.0) * g.sumL);
+            hipEventRecord(b->ev[ei], st);
             e = launch();
-            hipEventRecord(b->ev[ei + 1], b->ps[g.part]);
+            hipEventRecord(b->ev[ei + 1], st);
         } else {
             e = launch();
         }
@@ -1469,6 +1510,7 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
     if (!maxc) return 0;
     if (batch_upload(b, which, nullptr, count)) return -1;
     b->ev_stat.clear();
+    b->ev_bytes.clear();
     for (size_t s = 0; s < b->slots.size(); s++)
         if (b->slots[s].ctx) b->slots[s].ctx->eng->seq += b->h_slots[s].count;  // the table holds the old value
     auto launch_all = [&]() -> hipError_t {
@@ -1492,9 +1534,9 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
         if (replay) {
             // the other parts' streams join the capture (fork) and rejoin the first (join)
             hipEventRecord(b->evfork, b->stream);
-            for (int p = 1; p < b->nparts; p++) hipStreamWaitEvent(b->ps[p], b->evfork, 0);
+            for (int p = 1; p < b->nused; p++) hipStreamWaitEvent(b->ps[p], b->evfork, 0);
             const bool launched = launch_all() == hipSuccess;
-            for (int p = 1; p < b->nparts; p++) {
+            for (int p = 1; p < b->nused; p++) {
                 hipEventRecord(b->evjoin[p], b->ps[p]);
                 hipStreamWaitEvent(b->stream, b->evjoin[p], 0);
             }
@@ -1512,13 +1554,13 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
     if (replay && b->gexec) {
         // the graph runs every part on the first stream: order it after the other streams' work,
         // and their later work (copies) after it
-        for (int p = 1; p < b->nparts; p++) {
+        for (int p = 1; p < b->nused; p++) {
             BATCH_OK(b, hipEventRecord(b->evjoin[p], b->ps[p]));
             BATCH_OK(b, hipStreamWaitEvent(b->stream, b->evjoin[p], 0));
         }
         BATCH_OK(b, hipGraphLaunch(b->gexec, b->stream));
         BATCH_OK(b, hipEventRecord(b->evfork, b->stream));
-        for (int p = 1; p < b->nparts; p++) BATCH_OK(b, hipStreamWaitEvent(b->ps[p], b->evfork, 0));
+        for (int p = 1; p < b->nused; p++) BATCH_OK(b, hipStreamWaitEvent(b->ps[p], b->evfork, 0));
     } else {
         BATCH_OK(b, launch_all());
     }
@@ -1531,26 +1573,13 @@ extern "C" int kite_batch_square(kite_batch *b, const int32_t *which, const uint
             b->launches[p] = (maxc + b->timing - 1) / b->timing;
         }
         for (auto &ks : b->kstats) ks = kite_batch::KStat{ks.name, ks.pass};
-        std::vector<double> bytes(b->kstats.size(), 0.0);
-        for (const auto &g : b->grows)
-            if (g.count) {
-                bytes[g.stat[0]] += 16.0 * g.sumL;
-                if (g.kp) bytes[g.stat_p] += 16.0 * g.sumL;
-            }
-        for (const auto &g : b->gcols)
-            if (g.count) {
-                bytes[g.stat[0]] += 8.0 * g.sumL;
-                bytes[g.stat[1]] += 8.0 * g.sumL;
-                if (g.kf) bytes[g.stat_f] += 16.0 * g.sumL;
-                if (g.kp) bytes[g.stat_p] += 16.0 * g.sumL;
-            }
         for (size_t i = 0; i < b->ev_stat.size(); i++) {
             float ms = 0;
             hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]);
             kite_batch::KStat &ks = b->kstats[b->ev_stat[i]];
             ks.ms += ms;
             ks.launches++;
-            ks.bytes += bytes[b->ev_stat[i]];
+            ks.bytes += b->ev_bytes[i];
             b->ms[ks.pass] += ms;
         }
     }
@@ -1634,6 +1663,15 @@ extern "C" int kite_batch_set_xcds(kite_batch *b, int xcds) {
         b->dirty = true;
     }
     return xcds;
+}
+
+extern "C" int kite_batch_set_shape_parts(kite_batch *b, int on) {
+    on = on != 0;
+    if (on != b->shape_parts) {
+        b->shape_parts = on;
+        b->dirty = true;
+    }
+    return on;
 }
 
 extern "C" int kite_batch_set_persistent(kite_batch *b, int on) {

@@ -386,6 +386,9 @@ class RunOpts(ctypes.Structure):
         ("checkpoint_secs", c_dbl),
         ("order", c_u32),
         ("share", c_u32),
+        ("parts", c_u32),
+        ("shape_parts", c_u32),
+        ("xcds", c_u32),
     ]
 
 
@@ -395,6 +398,7 @@ SIGNATURES.update({
     "kite_batch_set_parts": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_set_xcds": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_batch_set_persistent": (ctypes.c_int, [c_p, ctypes.c_int]),
+    "kite_batch_set_shape_parts": (ctypes.c_int, [c_p, ctypes.c_int]),
     "kite_debug_fail_hip": (None, [ctypes.c_long]),
     "kite_job_bytes": (c_u64, [c_u32, c_u32]),
     "kite_batch_run": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(RunOpts),
@@ -404,7 +408,7 @@ SIGNATURES.update({
 
 def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fault_job=0, fault_at=0,
               max_roundoff=0.0, on_done=None, checkpoint_dir=None, checkpoint_secs=0.0, order=ORDER_ARRIVAL,
-              share=SHARE_SQUARINGS):
+              share=SHARE_SQUARINGS, parts=0, shape_parts=0, xcds=0):
     """Runs jobs [(kind, k, n, arrival), ...] (kind JOB_PRP or JOB_LLR; Fermat base 3) through
     kite_batch_run, with checkpoint files in checkpoint_dir if given. Returns (rc, [JobResult])."""
     arr = (Job * len(jobs))(*[Job(k, n, kind, 3, 0, arrival) for kind, k, n, arrival in jobs])
@@ -419,6 +423,9 @@ def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fa
     o.inject_fault_at = fault_at
     o.order = order
     o.share = share
+    o.parts = parts
+    o.shape_parts = shape_parts
+    o.xcds = xcds
     if checkpoint_dir is not None:
         o.checkpoint_dir = os.fsencode(checkpoint_dir)
         o.checkpoint_secs = checkpoint_secs
@@ -517,6 +524,10 @@ class Batch:
     def graphs(self, on):
         """Replay multi-step square calls from a captured HIP graph (the default) or launch directly."""
         self.lib.kite_batch_set_graphs(self.p, int(bool(on)))
+
+    def shape_parts(self, on):
+        """Each FFT shape but the most common on a stream of its own (kite_batch_set_shape_parts)."""
+        return self.lib.kite_batch_set_shape_parts(self.p, int(bool(on)))
 
     def persistent(self, on):
         """Persistent launches of the square-mode passes (kite_batch_set_persistent)."""

@@ -7,6 +7,7 @@ workload itself:
 
   orders   the shape variant (R x C split, column group width, kernel set) of each FFT length the
            workload uses, i.e. the "order" column of the tuning file ("L us order")
+  shape_parts  each FFT shape but the most common on a stream of its own (kite_batch_set_shape_parts)
   parts    streams the tests are dealt to (concurrent launches)
   xcds     XCDs the column jobs are laid out for (1: slot order)
   batch    tests in flight (the rest of a larger workload would wait for a slot)
@@ -15,7 +16,8 @@ Each trial is a fresh process (the tuning file is read once per process) that sq
 for `--seconds` after a warm-up, in square calls of `--block` steps replayed from the captured
 graph, and reports tests/hour. The search is coordinate-wise from the defaults (orders 0, 2
 parts, 8 XCDs, 384 tests): orders per length, then parts, XCDs and batch size, each kept only if it
-measures more than `--margin` percent faster; the winner is then measured twice more against the
+measures more than `--margin` percent faster (shape parts are tried before the stream count, so that
+the count is tuned with them); the winner is then measured twice more against the
 defaults, and kept only if it still wins by the margin. The result is written as JSON (`--out`); bench.py applies it when given
 `--tuned FILE` (the committed ahnum_amd/tuned/single_large.json by default).
 
@@ -31,7 +33,7 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-DEFAULT = {"orders": {}, "parts": 2, "xcds": 8, "batch": 384}
+DEFAULT = {"orders": {}, "shape_parts": 0, "parts": 2, "xcds": 8, "batch": 384}
 
 
 def fft_times_file(orders):
@@ -58,6 +60,7 @@ def trial_main(argv):
     ap.add_argument("--workload", default="single_large")
     ap.add_argument("--batch", type=int, default=384)
     ap.add_argument("--parts", type=int, default=2)
+    ap.add_argument("--shape-parts", type=int, default=0)
     ap.add_argument("--xcds", type=int, default=8)
     ap.add_argument("--block", type=int, default=200)
     ap.add_argument("--seconds", type=float, default=3.0)
@@ -68,6 +71,7 @@ def trial_main(argv):
     b = kite.Batch()
     try:
         b.parts(a.parts)
+        b.shape_parts(a.shape_parts)
         b.xcds(a.xcds)
         lengths = set()
         for i, (k, n) in enumerate(tests):
@@ -96,6 +100,7 @@ def trial_main(argv):
 def run_trial(cfg, workload, seconds, block):
     cmd = [sys.executable, "-m", "ahnum_amd.tune_batch", "--trial", "--workload", workload,
            "--batch", str(cfg["batch"]), "--parts", str(cfg["parts"]), "--xcds", str(cfg["xcds"]),
+           "--shape-parts", str(cfg.get("shape_parts", 0)),
            "--block", str(block), "--seconds", str(seconds)]
     env = apply_env(cfg)
     try:
@@ -141,7 +146,7 @@ def main(argv=None):
             r = measure(cfg, f"order L={L} -> {o}")
             if r["tests_per_hour"] > best_tph * (1 + a.margin / 100):
                 best, best_tph = cfg, r["tests_per_hour"]
-    for key, values in (("parts", [1, 2, 3, 4]), ("xcds", [1, 8]),
+    for key, values in (("shape_parts", [0, 1]), ("parts", [1, 2, 3, 4]), ("xcds", [1, 8]),
                         ("batch", [int(v) for v in a.batches.split(",")])):
         for v in values:
             if v == best[key]:

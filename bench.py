@@ -235,7 +235,7 @@ def serve_only(a):
 
 # ---- the Gerbicz-checked production loop ---------------------------------------------------------
 
-def gerbicz_leg(tests, secs, blocks):
+def gerbicz_leg(tests, secs, blocks, launch):
     """The tests through kite_batch_run for `secs`: Fermat from x_0 = 3^k, Gerbicz blocks of 1000
     squarings with their multiplies, a check every `blocks` blocks, round-off read every round.
     Rates start once every test has been admitted (their starts, x_0 = 3^k by powering, come
@@ -245,7 +245,7 @@ def gerbicz_leg(tests, secs, blocks):
     from ahnum_amd import kite
     jobs = [(kite.JOB_PRP, k, n, 0.0) for k, n in tests]
     t0 = time.perf_counter()
-    rc, out = kite.batch_run(jobs, block=1000, blocks=blocks, horizon=secs)
+    rc, out = kite.batch_run(jobs, block=1000, blocks=blocks, horizon=secs, **launch)
     wall = time.perf_counter() - t0
     admitted = max(r.start for r in out)  # the last test's setup done: all are in the rounds
     span = max(min(wall, secs) - admitted, 1e-9)
@@ -301,6 +301,9 @@ def main():
     ap.add_argument("--block", type=int, default=1000, help="squarings per square call (a Gerbicz block)")
     ap.add_argument("--parts", type=int, default=None,
                     help="concurrent streams the batch is dealt to (default: the tuned value, else 2)")
+    ap.add_argument("--shape-parts", type=int, default=None,
+                    help="each FFT shape but the most common on a stream of its own (default: the tuned value, "
+                         "else 0)")
     ap.add_argument("--xcds", type=int, default=None,
                     help="XCDs the column jobs are laid out for (default: the tuned value, else 8)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -390,6 +393,7 @@ def main():
     # ---- batched engine ----
     b = kite.Batch()
     parts = b.parts(a.parts)
+    shape_parts = b.shape_parts(int(tuned.get("shape_parts", 0)) if a.shape_parts is None else a.shape_parts)
     b.xcds(a.xcds)
     Ls = []
     for i, (k, n) in enumerate(tests):
@@ -458,8 +462,9 @@ def main():
             "workload": described,
             "batch": len(tests),
             "fft_lengths": sorted(set(Ls)),
-            "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams, "
-                      f"column jobs laid out for {a.xcds} XCDs",
+            "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams"
+                      + (" for the most common FFT shape and one for each other" if shape_parts else "")
+                      + f", column jobs laid out for {a.xcds} XCDs",
             "tuned": ({"file": os.path.relpath(tuned_path, ROOT), **tuned} if tuned_path else None),
             "value_is": "squaring throughput of the timed steps as tests/hour (sum of squarings/s * 3600 / n); "
                         "the production loop with Gerbicz checks is timed apart in gerbicz_checked",
@@ -500,7 +505,8 @@ def main():
         "gpu_ms_per_timed_step": round(sum(ms) / max(launches[0], 1), 4),
     }
     if a.gerbicz_secs > 0:
-        g = gerbicz_leg(tests, a.gerbicz_secs, a.gerbicz_blocks)
+        g = gerbicz_leg(tests, a.gerbicz_secs, a.gerbicz_blocks,
+                        {"parts": a.parts, "shape_parts": shape_parts, "xcds": a.xcds})
         g["vs_value"] = round(g["tests_per_hour"] / out["value"], 4) if out["value"] else None
         out["gerbicz_checked"] = g
     if a.serve_secs > 0 and a.workload == "single_large":

@@ -41,6 +41,12 @@ int kite_batch_set_parts(kite_batch *b, int parts);
 // MI355X, the default): each test's column groups run on one XCD. 1 keeps plain slot order.
 // Returns the value in use.
 int kite_batch_set_xcds(kite_batch *b, int xcds);
+// Shape parts (off by default): the slots of the most common shape (R x C split) are dealt to the
+// `parts` streams as usual, and each other shape's slots to a stream of its own while streams are
+// left (4 in all): a few tests of another FFT length then run their short launches beside the
+// main ones instead of after them in the same stream, and each length keeps its own column kernel
+// instead of sharing a two-length one. Returns the setting in use.
+int kite_batch_set_shape_parts(kite_batch *b, int on);
 // Persistent launches (off by default): the square-mode row and fused column passes run as
 // persistent kernels where those exist, on at most the device's resident capacity divided among
 // the parts, each work-group running a fixed share of the jobs, instead of one work-group per job.
@@ -186,6 +192,12 @@ typedef struct {
     // flight: every job gets about the same device time per round, so a short test is not held to
     // the pace of the longest ones in flight, which lowers serving latency).
     uint32_t share;
+    // The batch's launch settings (0 = its defaults): streams (kite_batch_set_parts), shape parts
+    // (kite_batch_set_shape_parts: 1 on) and the XCD layout (kite_batch_set_xcds), as a batched
+    // tuning run (python -m ahnum_amd.tune_batch) found best for the workload.
+    uint32_t parts;
+    uint32_t shape_parts;
+    uint32_t xcds;
 } kite_run_opts;
 
 // Device bytes a job of this kind holds at length L while in flight (the admission estimate).

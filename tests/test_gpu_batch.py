@@ -235,17 +235,20 @@ def test_hbm_budget_limits_jobs_in_flight():
     assert all("%016X" % r.res64 == "79291CD2F47CE48C" for r in out)
 
 
-@pytest.mark.parametrize("parts,graphs", [(1, True), (2, False), (3, True), (4, True)])
-def test_parts_streams_ordering(parts, graphs):
-    """Slots dealt to several streams: single-step and multi-step squarings, multiplies (whose
-    launches read the slot table uploaded on the first stream) and a slot removed and re-added
-    between calls; every value exact."""
+@pytest.mark.parametrize("parts,graphs,shape_parts", [(1, True, 0), (2, False, 0), (3, True, 0), (4, True, 0),
+                                                     (1, False, 1), (2, True, 1), (3, True, 1)])
+def test_parts_streams_ordering(parts, graphs, shape_parts):
+    """Slots dealt to several streams (with shape parts: the most common shape's slots to `parts`
+    streams and every other shape to a stream of its own while there are streams left):
+    single-step and multi-step squarings, multiplies (whose launches read the slot table uploaded
+    on the first stream) and a slot removed and re-added between calls; every value exact."""
     import ctypes
-    cases = [(9357, 20000), (8191, 25000), (3, 26459), (7, 4001), (4523, 67231)]
+    cases = [(9357, 20000), (8191, 25000), (3, 26459), (7, 4001), (4523, 67231), (9357, 20011)]
     b = kite.Batch()
     r = random.Random(parts)
     try:
         assert b.parts(parts) == parts
+        assert b.shape_parts(shape_parts) == shape_parts
         b.graphs(graphs)
         vals = {}
         for k, n in cases:
