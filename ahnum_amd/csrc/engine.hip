@@ -114,6 +114,9 @@ static void root_of_unity(uint64_t j, uint64_t n, double *c, double *s) {
     *s = (double)ss;
 }
 
+// Testing hook: the root of unity every device twiddle table is built from.
+extern "C" void kite_debug_root_of_unity(uint64_t j, uint64_t n, double *c, double *s) { root_of_unity(j, n, c, s); }
+
 // forward roots w_n^t = e^{-2 pi i t/n}
 static std::vector<double2> roots_table(uint64_t n, uint64_t count, uint64_t step) {
     std::vector<double2> t(count);

@@ -99,6 +99,9 @@ int kite_square_n_static(kite_ctx *ctx, kite_num *x, uint64_t count);
 // reported as failed (n <= 0: off). Nothing fails on the device; the library takes its error path:
 // the context's round-off reads +inf, batch calls return -1 and the drivers a negative code.
 void kite_debug_fail_hip(long n);
+// Testing hook (no device needed): cos and sin(2 pi j / n) as the library computes them for its
+// twiddle tables (exact integer reduction of j/n to [0, 1/8], long double evaluation).
+void kite_debug_root_of_unity(uint64_t j, uint64_t n, double *c, double *s);
 
 // ---- batched Fermat PRP tests ---------------------------------------------------------------
 
