@@ -8,6 +8,9 @@
                                                      # each appended to _bin/batch_more.partial
   python tests/golden/make_golden.py batch_merge     # res64_batch_subset.csv += finished rows
   python tests/golden/make_golden.py single_large    # the seeded n~660K test
+  python tests/golden/make_golden.py record_chain    # x_i of the record-size number (i = 50, 2000,
+                                                     # 10000; GMP ~0.35 s per squaring at 25M bits),
+                                                     # appended to chain_x.csv
 
 GMP (mpz_powm, the semantics of the reference's tools/gmp_ref.c) runs through gmp_res64.c,
 built here against the system libgmp. Outputs are CSV lines "k,n,base,RES64" (or
@@ -101,6 +104,16 @@ def main():
         out = [rows[key] for key in order if key in rows]
         write("res64_batch_subset.csv", f"# k,n,base,RES64 (GMP): {len(out)} of the 256 tests of "
               "workloads.batch_mixed(256, seed=3), in batch order", out)
+    elif what == "record_chain":
+        (k, n), = workloads.record()
+        rows = gmp_lines([f"{k} {n} 3 x {i}" for i in (50, 2000, 10000)], jobs)
+        have = read_rows("chain_x.csv")
+        keep = [r for r in have if not r.startswith(f"{k},{n},")]
+        path = os.path.join(HERE, "chain_x.csv")
+        head = [l.rstrip("\n") for l in open(path) if l.startswith("#")]
+        with open(path, "w") as f:
+            f.write("\n".join(head + keep + rows) + "\n")
+        print("wrote", path, len(keep + rows), "rows")
     elif what == "single_large":
         rows = gmp_lines([f"{k} {n} 3" for k, n in workloads.single_large()], 1,
                          partial=os.path.join(HERE, "_bin", "single_large.partial"))
