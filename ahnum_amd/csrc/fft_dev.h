@@ -740,10 +740,23 @@ __device__ __forceinline__ int tid_x() {
 // strided writes of the early Stockham stages over distinct LDS bank groups.
 template <int PAD> __device__ __forceinline__ int padi(int i) { return PAD ? i + (i >> PAD) : i; }
 
+// Radix plan of fft_ip: lengths of 1024 and more start with radix-16 stages (1024 = 16 x 8 x 8,
+// 2048 = 16 x 16 x 8), then the plan of the other kernels: three stages instead of four, without
+// the radix-2 stage (N/2 butterflies) of 8 x 8 x 8 x 2.
+__host__ __device__ constexpr int ip_lead16(int n) {
+    return n >= 2048 && n % 256 == 0 ? 2 : n >= 1024 && n % 16 == 0 ? 1 : 0;
+}
+__host__ __device__ constexpr int ip_nstages(int n) {
+    return ip_lead16(n) + ct_nstages(n >> (4 * ip_lead16(n)));
+}
+__host__ __device__ constexpr int ip_radix(int n, int idx) {
+    return idx < ip_lead16(n) ? 16 : ct_radix(n >> (4 * ip_lead16(n)), idx - ip_lead16(n));
+}
+
 template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV, int PAD = 0>
 __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
-    if constexpr (I < ct_nstages(N)) {
-        constexpr int P = ct_radix(N, I), NB = N / P, TOTAL = NB * HOW;
+    if constexpr (I < ip_nstages(N)) {
+        constexpr int P = ip_radix(N, I), NB = N / P, TOTAL = NB * HOW;
         constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
         double2 v[ITER][P];
         int ob[ITER];
@@ -789,6 +802,13 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
 // LDS passes and barriers of the column lengths up to 128; else the plan of the other kernels.
 // For the column lengths up to 128 whose R/8 is no DFT length here, the most balanced split into
 // two DFT lengths (100 = 10 x 10, 36 = 6 x 6, 120 = 12 x 10), else the plan of the other kernels.
+// Longer columns, and the shorter ones without such a split (io_long), where the plan of the other
+// kernels has a radix-2 stage or could have fewer stages: the fewest stages of radices up to
+// kIoMaxRadix, then the smallest largest radix, then the fewest butterflies (sum of N/P), radices
+// in descending order (108 = 6 x 6 x 3, 144 = 6 x 6 x 4, 216 = 6 x 6 x 6, 400 = 10 x 8 x 5). The
+// radix stays small because the fused column pass is latency-bound: more, shorter butterflies per
+// stage beat fewer stages (144 as 12 x 12 and 192 as 16 x 12 measured 10-15% slower).
+constexpr int kIoMaxRadix = 10;
 __host__ __device__ constexpr bool dft_len(int n) {
     return n == 2 || n == 3 || n == 4 || n == 5 || n == 6 || n == 7 || n == 8 || n == 10 || n == 12 || n == 14 ||
            n == 16;
@@ -808,9 +828,49 @@ __host__ __device__ constexpr int io_first(int n) {
     return best;
 }
 __host__ __device__ constexpr bool io_two(int n) { return io_first(n) != 0; }
-__host__ __device__ constexpr int io_nstages(int n) { return io_two(n) ? 2 : ct_nstages(n); }
+struct IoSplit {
+    int ns, r[4];
+};
+__host__ __device__ constexpr bool ct_has2(int n) {
+    for (int i = 0; i < ct_nstages(n); i++)
+        if (ct_radix(n, i) == 2) return true;
+    return false;
+}
+__host__ __device__ constexpr IoSplit io_long(int n) {
+    IoSplit best{0, {0, 0, 0, 0}};
+    int bmax = 0, bsum = 0;
+    auto take = [&](int ns, int a, int b, int c, int d) {
+        const int sum = n / a + n / b + (c ? n / c : 0) + (d ? n / d : 0);
+        if (best.ns == 0 || ns < best.ns || (ns == best.ns && (a < bmax || (a == bmax && sum < bsum)))) {
+            best = IoSplit{ns, {a, b, c, d}};
+            bmax = a;
+            bsum = sum;
+        }
+    };
+    auto ok = [](int r) { return r >= 2 && r <= kIoMaxRadix && dft_len(r); };
+    for (int a = kIoMaxRadix; a >= 2; a--) {
+        if (!ok(a) || n % a) continue;
+        const int m = n / a;
+        if (m <= a && ok(m)) take(2, a, m, 0, 0);
+        for (int b = a; b >= 2; b--) {
+            if (!ok(b) || m % b) continue;
+            const int m2 = m / b;
+            if (m2 <= b && ok(m2)) take(3, a, b, m2, 0);
+            for (int c = b; c >= 2; c--)
+                if (ok(c) && m2 % c == 0 && m2 / c <= c && ok(m2 / c)) take(4, a, b, c, m2 / c);
+        }
+    }
+    // only where it beats the plan of the other kernels: fewer stages, or no radix-2 stage (which
+    // has N/2 butterflies)
+    if (best.ns && best.ns >= ct_nstages(n) && !ct_has2(n)) best.ns = 0;
+    return best;
+}
+__host__ __device__ constexpr int io_nstages(int n) {
+    return io_two(n) ? 2 : io_long(n).ns ? io_long(n).ns : ct_nstages(n);
+}
 __host__ __device__ constexpr int io_radix(int n, int idx) {
-    return io_two(n) ? (idx == 0 ? io_first(n) : n / io_first(n)) : ct_radix(n, idx);
+    if (io_two(n)) return idx == 0 ? io_first(n) : n / io_first(n);
+    return io_long(n).ns ? io_long(n).r[idx] : ct_radix(n, idx);
 }
 
 struct NoPre {

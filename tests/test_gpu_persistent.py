@@ -1,7 +1,7 @@
 """Persistent launches (kite_batch_set_persistent): the square-mode row and fused column passes run
-on about the device's resident capacity, each work-group taking jobs by ticket. Every value must be
-exact and bit-identical to the one-work-group-per-job launches, whatever the streams, the XCD
-layout, idle slots and repeated replays (the tickets are cleared by each launch's last work-group)."""
+on about the device's resident capacity, each work-group walking a fixed list of jobs that the host
+wrote once. Every value must be exact and bit-identical to the one-work-group-per-job launches,
+whatever the streams, the XCD layout, idle slots and repeated graph replays."""
 import ctypes
 import random
 
@@ -50,7 +50,7 @@ def test_persistent_exact_and_same_as_direct(parts, xcds):
         for b in (bp, bd):
             b.square(5)  # a captured graph of 5 steps
             b.square(1)  # direct launches
-            b.square(5)  # the graph replayed (the tickets were cleared by every launch)
+            b.square(5)  # the graph replayed
         for s, (k, n, a) in vals.items():
             got = bp.get(s, 0)
             assert got == bd.get(s, 0), (k, n)
