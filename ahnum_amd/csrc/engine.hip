@@ -306,6 +306,7 @@ struct Engine {
     size_t lds_rows1 = 0;
     PassKernel kcfused = nullptr;   // square-mode fused column pass, if any
     int nt_fused = kNT;             // its threads per work-group
+    int nt_rows = kNT;              // threads per work-group of krows and krows1
     unsigned long long *flags = nullptr;  // its per-chunk carry flags
     uint32_t seq = 0;                     // its launches so far
     SlotDev *d_slot = nullptr;            // the slot table of this context's own launches (one entry)
@@ -492,6 +493,7 @@ int engine_init(kite_ctx *ctx) {
         E->kcfused = fused_ok ? fast_cols_fused(sh.R) : nullptr;
         E->lds_fused = fused_ok ? fast_cols_fused_lds(sh.R) : 0;
         E->nt_fused = fused_ok ? fast_cols_fused_threads(sh.R) : kNT;
+        E->nt_rows = fr ? rows_nt(sh.C) : kNT;
         E->name[0] = E->krows_sq ? "k_rows_sq" + std::to_string(sh.C)
                      : fr       ? "k_rows_c<" + std::to_string(sh.C) + ">"
                                 : std::string("k_rows_g");
@@ -585,7 +587,8 @@ static bool launch_rows(Engine *E, int mode, int step = 0, const SlotDev *tbl = 
     if (mode == kModeSquare && E->krows_sq)
         return chk(E, launch_pass(E->krows_sq, E->nrowjobs, 0, E->stream, tbl, nullptr, mode, step, kRowsSqThreads),
                    "rows pass");
-    return chk(E, launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, tbl, nullptr, mode, step), "rows pass");
+    return chk(E, launch_pass(E->krows, E->nrowjobs, E->lds_rows, E->stream, tbl, nullptr, mode, step, E->nt_rows),
+               "rows pass");
 }
 
 static bool launch_cols(Engine *E, int mode, int step = 0, const SlotDev *tbl = nullptr) {
@@ -603,7 +606,8 @@ static bool launch_square(Engine *E, int step, const SlotDev *tbl = nullptr) {
     if (!tbl) tbl = E->d_slot;
     bool ok;
     if (E->krows1)
-        ok = chk(E, launch_pass(E->krows1, E->nrowjobs1, E->lds_rows1, E->stream, tbl, nullptr, kModeSquare, step),
+        ok = chk(E, launch_pass(E->krows1, E->nrowjobs1, E->lds_rows1, E->stream, tbl, nullptr, kModeSquare, step,
+                                E->nt_rows),
                  "rows pass");
     else
         ok = launch_rows(E, kModeSquare, step, tbl);
@@ -871,6 +875,7 @@ struct kite_batch {
         int stat_f = -1;
         size_t lds_f = 0;
         int nt_f = kite::kNT;  // its threads per work-group
+        int nt_r = kite::kNT;  // threads per work-group of the row kernels k1 (and k2 unless rsq)
         int part = 0;  // the stream the group's launches go to
         // the persistent variant of the square-mode pass (rows: k2, columns: kf), if any; its job
         // lists (index into d_lists, the list count, the longest list) and the work-groups each list
@@ -1095,6 +1100,7 @@ static int batch_refresh(kite_batch *b) {
                 g.stat[0] = stat_of(rk0 == E0->krows1 && alone ? "k_rows_c1<" + std::to_string(E0->shape->ch.C) + ">"
                                                                : E0->name[0], 0);
                 g.kp = rsq0 ? fast_rows_sq_p(E0->shape->ch.C) : nullptr;
+                g.nt_r = E0->nt_rows;
                 if (g.kp) g.stat_p = stat_of(E0->name[0] + "_p", 0);
             } else {
                 g.stat[0] = stat_of(E0->name[1], 1);
@@ -1344,7 +1350,7 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
         if (fused && pass == 2) continue;
         const PassKernel fn = fused ? g.kf : (pass == 2 || rsq) ? g.k2 : g.k1;
         const size_t lds = rsq ? 0 : fused ? g.lds_f : g.lds;
-        const int threads = rsq ? kRowsSqThreads : fused ? g.nt_f : kNT;
+        const int threads = rsq ? kRowsSqThreads : fused ? g.nt_f : pass == 0 ? g.nt_r : kNT;
         const int pgrid = b->persist && g.kp && (rsq || fused) ? persist_grid(b, g, threads, lds) : 0;
         const bool persist = pgrid > 0;
         // a call with timing on runs every launch on the first stream, so that each timed launch

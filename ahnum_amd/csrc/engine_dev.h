@@ -55,6 +55,9 @@ __host__ __device__ constexpr int g_for_R(int R) {
 // Row pairs per work-group for row length C in the compile-time row kernels (~2048 elements, so
 // that every radix-8 stage keeps all threads busy).
 __host__ __device__ constexpr int ppw_for_C(int C) { return C >= 1024 ? 1 : 1024 / C; }
+// Threads per work-group of the LDS row kernels: 512 at C = 2048 (one row pair of 4096 elements:
+// each radix-16 or radix-8 stage in one round of butterflies), else kNT.
+__host__ __device__ constexpr int rows_nt(int C) { return C >= 2048 ? 512 : kNT; }
 
 enum : int { kModeSquare = 0, kModeMul = 1, kModeDigits = 2, kModeRowFft = 3 };
 

@@ -753,17 +753,17 @@ __host__ __device__ constexpr int ip_radix(int n, int idx) {
     return idx < ip_lead16(n) ? 16 : ct_radix(n >> (4 * ip_lead16(n)), idx - ip_lead16(n));
 }
 
-template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV, int PAD = 0>
+template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV, int PAD = 0, int NT = kNT>
 __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
     if constexpr (I < ip_nstages(N)) {
         constexpr int P = ip_radix(N, I), NB = N / P, TOTAL = NB * HOW;
-        constexpr int ITER = (TOTAL + kNT - 1) / kNT, TS = N / (NS * P);
+        constexpr int ITER = (TOTAL + NT - 1) / NT, TS = N / (NS * P);
         double2 v[ITER][P];
         int ob[ITER];
 #pragma unroll
         for (int it = 0; it < ITER; it++) {
-            const int w = it * kNT + (int)threadIdx.x;
-            if (TOTAL % kNT == 0 || w < TOTAL) {
+            const int w = it * NT + (int)threadIdx.x;
+            if (TOTAL % NT == 0 || w < TOTAL) {
                 const int seq = SEQFAST ? w % HOW : w / NB, j = SEQFAST ? w / HOW : w % NB;
                 const int jm = j % NS;
 #pragma unroll
@@ -782,14 +782,14 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITER; it++) {
-            const int w = it * kNT + (int)threadIdx.x;
-            if (TOTAL % kNT == 0 || w < TOTAL) {
+            const int w = it * NT + (int)threadIdx.x;
+            if (TOTAL % NT == 0 || w < TOTAL) {
 #pragma unroll
                 for (int r = 0; r < P; r++) a[padi<PAD>(ob[it] + r * NS * IS)] = v[it][r];
             }
         }
         __syncthreads();
-        fft_ip<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV, PAD>(a, tw);
+        fft_ip<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV, PAD, NT>(a, tw);
     }
 }
 
@@ -962,7 +962,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
     const SlotDev &s = slots[jb.x];
     if (step >= (int)s.count) return;
-    constexpr int PPW = PPW_T, NROW = 2 * PPW, NE = NROW * C;
+    constexpr int PPW = PPW_T, NROW = 2 * PPW, NE = NROW * C, NT = rows_nt(C);
     constexpr bool TWL = C <= kTwLdsMaxC;
     constexpr int PD = kRowPad;  // row buffer index i lives at i + i/8
     const int R = s.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
@@ -970,7 +970,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
     const double2 *twC = s.twC;
     if (TWL) {
         double2 *T = lds + padi<PD>(NE);
-        for (int i = threadIdx.x; i < C; i += kNT) T[i] = gld(s.twC + i);
+        for (int i = threadIdx.x; i < C; i += NT) T[i] = gld(s.twC + i);
         twC = T;
     }
     auto row_of = [&](int i, int h) -> int {
@@ -980,28 +980,28 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
         return h == 0 ? p : R - p;
     };
     const double2 *x = mode == kModeRowFft ? s.y_src : s.x;
-    for (int e = threadIdx.x; e < NE; e += kNT) {
+    for (int e = threadIdx.x; e < NE; e += NT) {
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
         A[padi<PD>(e)] = row >= 0 ? gld(x + (size_t)row * C + c) : make_double2(0, 0);
     }
     // split items: one per (local pair, k_c); the twiddle of item kc of pair p > 0 (and of row 0 of
     // pair 0) is fetched before the FFT
-    constexpr int NIT = PPW * C, ITEMS = (NIT + kNT - 1) / kNT;
+    constexpr int NIT = PPW * C, ITEMS = (NIT + NT - 1) / NT;
     const bool split = mode == kModeSquare || mode == kModeMul;
     double2 tw_[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const int w = it * kNT + (int)threadIdx.x;
+        const int w = it * NT + (int)threadIdx.x;
         const int i = w / C, kc = w - i * C, p = jb.y * PPW + i;
-        const bool need = split && (NIT % kNT == 0 || w < NIT) && p < npairs;
+        const bool need = split && (NIT % NT == 0 || w < NIT) && p < npairs;
         if constexpr (rows_tw2<C>()) tw_[it] = need ? tw_h2(s, p + R * kc) : make_double2(1, 0);
         else tw_[it] = need ? gld(s.twRows + (size_t)p * C + kc) : make_double2(1, 0);
     }
     __syncthreads();
-    fft_ip<C, 1, 0, NROW, 1, C, false, false, PD>(A, twC);
+    fft_ip<C, 1, 0, NROW, 1, C, false, false, PD, NT>(A, twC);
     if (mode == kModeRowFft) {  // operand preparation: store the row spectra
-        for (int e = threadIdx.x; e < NE; e += kNT) {
+        for (int e = threadIdx.x; e < NE; e += NT) {
             const int slot = e / C, c = e - slot * C;
             const int row = row_of(slot >> 1, slot & 1);
             if (row >= 0) gst(s.yfreq + (size_t)row * C + c, A[padi<PD>(e)]);
@@ -1011,8 +1011,8 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
     const bool mul = mode == kModeMul;
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const int w = it * kNT + (int)threadIdx.x;
-        if (NIT % kNT != 0 && w >= NIT) continue;
+        const int w = it * NT + (int)threadIdx.x;
+        if (NIT % NT != 0 && w >= NIT) continue;
         const int i = w / C, kc = w - i * C, p = jb.y * PPW + i;
         if (p >= npairs) continue;
         if (p > 0) {  // (p, kc) with (R-p, C-1-kc)
@@ -1046,8 +1046,8 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
         }
     }
     __syncthreads();
-    fft_ip<C, 1, 0, NROW, 1, C, false, true, PD>(A, twC);
-    for (int e = threadIdx.x; e < NE; e += kNT) {
+    fft_ip<C, 1, 0, NROW, 1, C, false, true, PD, NT>(A, twC);
+    for (int e = threadIdx.x; e < NE; e += NT) {
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
         if (row >= 0) gst(s.x + (size_t)row * C + c, A[padi<PD>(e)]);

@@ -25,13 +25,13 @@
 namespace kite {
 
 template <int C>
-__global__ void __launch_bounds__(kNT) k_rows_c(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
-                                              int mode, int step) {
+__global__ void __launch_bounds__(rows_nt(C)) k_rows_c(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+                                                     int mode, int step) {
     rows_fast<C>(slots, jobs, mode, step);
 }
 // One row pair per work-group: a single test's row pass on four times as many work-groups.
 template <int C>
-__global__ void __launch_bounds__(kNT) k_rows_c1(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
+__global__ void __launch_bounds__(rows_nt(C)) k_rows_c1(const SlotDev *__restrict__ slots, const int2 *__restrict__ jobs,
                                                int mode, int step) {
     rows_fast<C, 1>(slots, jobs, mode, step);
 }
