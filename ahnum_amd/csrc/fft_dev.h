@@ -779,7 +779,9 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
                 ob[it] = ((j / NS) * NS * P + jm) * IS + seq * SS;
             }
         }
-        __syncthreads();
+        // in place: reads before writes, except in the last stage (NS P = N), where every thread
+        // writes back the elements it read
+        if constexpr (NS * P != N) __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITER; it++) {
             const int w = it * NT + (int)threadIdx.x;
@@ -930,7 +932,9 @@ __device__ __forceinline__ void fft_io(double2 *a, const double2 *tw, LD &ld, ST
                 }
             }
         } else {
-            if constexpr (!FROM_LD) __syncthreads();  // the stage is in place: reads before writes
+            // the stage is in place: reads before writes, except in the last stage, whose outputs
+            // (rows j + r NS, NS = N / P = NB) are the very elements each thread read
+            if constexpr (!FROM_LD && I != NST - 1) __syncthreads();
 #pragma unroll
             for (int it = 0; it < ITER; it++) {
                 const int w = it * NT + tid;
