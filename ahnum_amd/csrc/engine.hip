@@ -503,12 +503,11 @@ int engine_init(kite_ctx *ctx) {
         E->ncoljobs = sh.NG;
         if (fr && ppw > 1 && (E->krows1 = fast_rows1(sh.C))) {
             E->nrowjobs1 = npairs;
-            E->lds_rows1 = (size_t)16 * (2 * sh.C + ((2 * sh.C) >> kRowPad) + (sh.C <= (int)kTwLdsMaxC ? sh.C : 0));
+            E->lds_rows1 = rows_lds_bytes(sh.C, 1);
         }
         const int sub = sh.G < kSub ? sh.G : kSub;
         if (fr)  // single buffer of 2*ppw rows, then the row twiddles
-            E->lds_rows = (size_t)16 * (2 * ppw * sh.C + ((2 * ppw * sh.C) >> kRowPad) +
-                                         (sh.C <= (int)kTwLdsMaxC ? sh.C : 0));
+            E->lds_rows = rows_lds_bytes(sh.C, ppw);
         else
             E->lds_rows = (size_t)16 * (2 * 2 * sh.C);
         if (fi)

@@ -16,7 +16,7 @@ constexpr int kMaxLds = 160 * 1024; // gfx950 LDS per work-group
 constexpr int kMaxTop = 64;         // top-region digits handled by the device fold
 constexpr int kSub = 8;             // elements (16 digits) per carry sub-chunk
 constexpr int kTwLdsMaxC = 1024;    // row twiddles are staged in LDS up to this row length
-constexpr int kRowPad = 3;          // row-kernel LDS: one padding element per 8
+constexpr int kRowTwPad = 3;        // row-kernel LDS twiddle table: one padding element per 8
 constexpr int kXcds = 8;            // MI355X: 8 XCDs; work-group i of a launch runs on XCD i mod 8
 constexpr int kFusedMinBits = 6;    // fused column pass: digits of at least this many bits (carry < 2^47)
 constexpr int kRowsSqThreads = 128; // threads per work-group of the register row kernel (C = 256)
@@ -58,6 +58,17 @@ __host__ __device__ constexpr int ppw_for_C(int C) { return C >= 1024 ? 1 : 1024
 // Threads per work-group of the LDS row kernels: 512 at C = 2048 (one row pair of 4096 elements:
 // each radix-16 or radix-8 stage in one round of butterflies), else kNT.
 __host__ __device__ constexpr int rows_nt(int C) { return C >= 2048 ? 512 : kNT; }
+// LDS of the compile-time row kernels: 2 ppw rows of C with one padding element per 2^rows_pad(C)
+// (per 16 where the FFT starts with radix 16, else per 8: the first stage's strided writes then
+// spread over all banks), then, for C <= kTwLdsMaxC, the w_C table, for C >= 512 with one padding
+// element per 2^kRowTwPad (the stages' twiddle reads jm r TS, with strides of whole bank rows,
+// spread the same way: k_rows_c<1024> 61.7 -> 56.8 us; at C = 128 it measured 1.5% slower).
+__host__ __device__ constexpr int rows_pad(int C) { return C >= 1024 ? 4 : 3; }
+__host__ __device__ constexpr int rows_twpad(int C) { return C >= 512 && C <= (int)kTwLdsMaxC ? kRowTwPad : 0; }
+__host__ __device__ constexpr size_t rows_lds_bytes(int C, int ppw) {
+    return (size_t)16 * (2 * ppw * C + ((2 * ppw * C) >> rows_pad(C)) +
+                         (C <= (int)kTwLdsMaxC ? C + (rows_twpad(C) ? C >> rows_twpad(C) : 0) : 0));
+}
 
 enum : int { kModeSquare = 0, kModeMul = 1, kModeDigits = 2, kModeRowFft = 3 };
 

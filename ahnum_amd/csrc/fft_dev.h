@@ -753,7 +753,9 @@ __host__ __device__ constexpr int ip_radix(int n, int idx) {
     return idx < ip_lead16(n) ? 16 : ct_radix(n >> (4 * ip_lead16(n)), idx - ip_lead16(n));
 }
 
-template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV, int PAD = 0, int NT = kNT>
+// TPAD: the twiddle table is padded like the data (entry t at padi<TPAD>(t)).
+template <int N, int NS, int I, int HOW, int IS, int SS, bool SEQFAST, bool INV, int PAD = 0, int NT = kNT,
+          int TPAD = 0>
 __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
     if constexpr (I < ip_nstages(N)) {
         constexpr int P = ip_radix(N, I), NB = N / P, TOTAL = NB * HOW;
@@ -771,7 +773,7 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
                 if (NS > 1 && jm) {
 #pragma unroll
                     for (int r = 1; r < P; r++) {
-                        const double2 t = tw[jm * r * TS];
+                        const double2 t = tw[padi<TPAD>(jm * r * TS)];
                         v[it][r] = INV ? cmulc(v[it][r], t) : cmul(v[it][r], t);
                     }
                 }
@@ -791,7 +793,7 @@ __device__ __forceinline__ void fft_ip(double2 *a, const double2 *tw) {
             }
         }
         __syncthreads();
-        fft_ip<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV, PAD, NT>(a, tw);
+        fft_ip<N, NS * P, I + 1, HOW, IS, SS, SEQFAST, INV, PAD, NT, TPAD>(a, tw);
     }
 }
 
@@ -968,13 +970,13 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
     if (step >= (int)s.count) return;
     constexpr int PPW = PPW_T, NROW = 2 * PPW, NE = NROW * C, NT = rows_nt(C);
     constexpr bool TWL = C <= kTwLdsMaxC;
-    constexpr int PD = kRowPad;  // row buffer index i lives at i + i/8
+    constexpr int PD = rows_pad(C), TPD = rows_twpad(C);  // row buffer index i lives at padi<PD>(i)
     const int R = s.R, npairs = R % 2 == 0 ? R / 2 : (R + 1) / 2;
     double2 *A = lds;
     const double2 *twC = s.twC;
     if (TWL) {
         double2 *T = lds + padi<PD>(NE);
-        for (int i = threadIdx.x; i < C; i += NT) T[i] = gld(s.twC + i);
+        for (int i = threadIdx.x; i < C; i += NT) T[padi<TPD>(i)] = gld(s.twC + i);
         twC = T;
     }
     auto row_of = [&](int i, int h) -> int {
@@ -1003,7 +1005,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
         else tw_[it] = need ? gld(s.twRows + (size_t)p * C + kc) : make_double2(1, 0);
     }
     __syncthreads();
-    fft_ip<C, 1, 0, NROW, 1, C, false, false, PD, NT>(A, twC);
+    fft_ip<C, 1, 0, NROW, 1, C, false, false, PD, NT, TPD>(A, twC);
     if (mode == kModeRowFft) {  // operand preparation: store the row spectra
         for (int e = threadIdx.x; e < NE; e += NT) {
             const int slot = e / C, c = e - slot * C;
@@ -1050,7 +1052,7 @@ __device__ __forceinline__ void rows_fast(const SlotDev *__restrict__ slots, con
         }
     }
     __syncthreads();
-    fft_ip<C, 1, 0, NROW, 1, C, false, true, PD, NT>(A, twC);
+    fft_ip<C, 1, 0, NROW, 1, C, false, true, PD, NT, TPD>(A, twC);
     for (int e = threadIdx.x; e < NE; e += NT) {
         const int slot = e / C, c = e - slot * C;
         const int row = row_of(slot >> 1, slot & 1);
