@@ -153,7 +153,8 @@ def test_gpu_matches_oracle_digits_value():
 
 
 def test_chain_vectors_gmp(golden):
-    """x_i = 3^(k 2^i) mod N against GMP, up to n = 659321 (tests/golden/chain_x.csv)."""
+    """x_i = 3^(k 2^i) mod N against GMP (tests/golden/chain_x.csv), up to the record size
+    91*2^24767982-1 (L = 1638400) at i = 50, 2000 and 10000."""
     rows = golden("chain_x.csv")
     ctxs = {}
     try:
