@@ -46,10 +46,12 @@ __host__ __device__ constexpr int ct_radix(int n, int idx) {
         }
     return 0;
 }
-// Column group width for column length R: about 2048 elements per work-group, 4..64 columns.
+// Column group width for column length R: about 2048 elements per work-group, 4..64 columns; for
+// R 288..512, 8 columns (up to 4096 elements): 128-byte row segments, whole cache lines, where
+// 4 columns wrote half lines (the record's fused pass at R = 400: 40.6 -> 32.4 us).
 __host__ __device__ constexpr int g_for_R(int R) {
     int G = 64;
-    while (G > 4 && G * R > 2048) G /= 2;
+    while (G > 4 && G * R > (R > 256 ? 4096 : 2048)) G /= 2;
     return G;
 }
 // Row pairs per work-group for row length C in the compile-time row kernels (~2048 elements, so

@@ -48,9 +48,12 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_r(const SlotDev *__restrict__ 
 // The fused kernels run fused_nt<R>() threads per work-group, the pair kernels the larger of their
 // two lengths' counts (fewer would make the longer length iterate, which spilled registers);
 // fast_cols_fused_threads and fast_cols_pair_fused_threads tell the host.
+// Waves per SIMD the fused kernels are compiled for (__launch_bounds__): 4 (128 VGPRs); 2 where
+// the groups are 8 columns of R >= 384 (a single record-size test's pass: one work-group per CU).
+constexpr int fused_waves_eu(int R) { return g_for_R(R) * R > 2048 ? 2 : 4; }
 template <int R1, int R2> constexpr int fused_nt2() { return fused_nt<R1>() > fused_nt<R2>() ? fused_nt<R1>() : fused_nt<R2>(); }
 template <int R>
-__global__ void __launch_bounds__(fused_nt<R>(), 4) k_cols_fused_r(const SlotDev *__restrict__ slots,
+__global__ void __launch_bounds__(fused_nt<R>(), fused_waves_eu(R)) k_cols_fused_r(const SlotDev *__restrict__ slots,
                                                                  const int2 *__restrict__ jobs, int mode, int step) {
     (void)mode;
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
@@ -75,7 +78,7 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_rr(const SlotDev *__restrict__
     else cols_fwd_fast<R2>(slots, jobs, mode, step);
 }
 template <int R1, int R2>
-__global__ void __launch_bounds__((fused_nt2<R1, R2>()), 4) k_cols_fused_rr(const SlotDev *__restrict__ slots,
+__global__ void __launch_bounds__((fused_nt2<R1, R2>()), fused_waves_eu(R2)) k_cols_fused_rr(const SlotDev *__restrict__ slots,
                                                                    const int2 *__restrict__ jobs, int mode, int step) {
     (void)mode;
     const int2 jb = jobs ? jobs[blockIdx.x] : make_int2(0, (int)blockIdx.x);
@@ -116,14 +119,14 @@ __device__ __forceinline__ void persist_loop(const int2 *__restrict__ jobs, cons
 }
 
 template <int R>
-__global__ void __launch_bounds__(fused_nt<R>(), 4) k_cols_fused_r_p(const SlotDev *__restrict__ slots,
+__global__ void __launch_bounds__(fused_nt<R>(), fused_waves_eu(R)) k_cols_fused_r_p(const SlotDev *__restrict__ slots,
                                                                    const int2 *__restrict__ jobs, int step,
                                                                    const unsigned int *lists, int nl) {
     persist_loop(jobs, lists, nl,
                  [&](int2 jb) { cols_fused_fast<R, true, fused_nt<R>()>(slots[jb.x], jb.y, step); });
 }
 template <int R1, int R2>
-__global__ void __launch_bounds__((fused_nt2<R1, R2>()), 4) k_cols_fused_rr_p(const SlotDev *__restrict__ slots,
+__global__ void __launch_bounds__((fused_nt2<R1, R2>()), fused_waves_eu(R2)) k_cols_fused_rr_p(const SlotDev *__restrict__ slots,
                                                                      const int2 *__restrict__ jobs, int step,
                                                                      const unsigned int *lists, int nl) {
     persist_loop(jobs, lists, nl, [&](int2 jb) {

@@ -287,7 +287,9 @@ def test_llr_impossible_roundoff_limit():
 # has that column length (C = 4R or so), squared through the batch's fused column kernel
 ODD_PART_LENGTHS = [(36, 9216), (60, 30720), (72, 36864), (100, 51200), (108, 55296), (120, 122880),
                     (144, 147456), (200, 409600), (216, 442368), (240, 491520), (288, 589824),
-                    (400, 1638400), (432, 1769472), (480, 1966080), (576, 2359296)]
+                    (400, 1638400), (432, 1769472), (480, 1966080), (576, 2359296),
+                    # and the other lengths R 288..512 of 8-column groups (C = 2048)
+                    (320, 1310720), (384, 1572864), (448, 1835008), (512, 2097152)]
 
 
 @pytest.mark.parametrize("R,L", ODD_PART_LENGTHS)
