@@ -10,7 +10,9 @@ workload itself:
   shape_parts  each FFT shape but the most common on a stream of its own (kite_batch_set_shape_parts)
   parts    streams the tests are dealt to (concurrent launches)
   xcds     XCDs the column jobs are laid out for (1: slot order)
-  batch    tests in flight (the rest of a larger workload would wait for a slot)
+  batch    tests in flight (the rest of a larger workload would wait for a slot); single_large
+           only: the other workloads are fixed sets of tests (batch_mixed 256, large_fft 32,
+           plain_small 16), all in flight
 
 Each trial is a fresh process (the tuning file is read once per process) that squares the batch
 for `--seconds` after a warm-up, in square calls of `--block` steps replayed from the captured
@@ -18,10 +20,12 @@ graph, and reports tests/hour. The search is coordinate-wise from the defaults (
 parts, 8 XCDs, 384 tests): orders per length, then parts, XCDs and batch size, each kept only if it
 measures more than `--margin` percent faster (shape parts are tried before the stream count, so that
 the count is tuned with them); the winner is then measured twice more against the
-defaults, and kept only if it still wins by the margin. The result is written as JSON (`--out`); bench.py applies it when given
-`--tuned FILE` (the committed ahnum_amd/tuned/single_large.json by default).
+defaults, and kept only if it still wins by the margin. The result is written as JSON (`--out`,
+default ahnum_amd/tuned/<workload>.json); bench.py applies the committed file of the workload it
+runs (or the one given with `--tuned FILE`).
 
-Usage: python -m ahnum_amd.tune_batch [--workload single_large] [--out FILE] [--seconds 3]
+Usage: python -m ahnum_amd.tune_batch [--workload single_large|batch_mixed|large_fft] [--out FILE]
+       [--seconds 3]
 """
 import argparse
 import json
@@ -34,6 +38,14 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 DEFAULT = {"orders": {}, "shape_parts": 0, "parts": 2, "xcds": 8, "batch": 384}
+# workloads that are a fixed set of tests (all in flight): their batch is the set's size, not tuned
+FIXED_BATCH = {"batch_mixed": 256, "large_fft": 32, "plain_small": 16}
+
+
+def defaults(workload):
+    d = json.loads(json.dumps(DEFAULT))
+    d["batch"] = FIXED_BATCH.get(workload, d["batch"])
+    return d
 
 
 def fft_times_file(orders):
@@ -67,7 +79,7 @@ def trial_main(argv):
     a = ap.parse_args(argv)
     sys.path.insert(0, ROOT)
     from ahnum_amd import kite, workloads
-    tests = getattr(workloads, a.workload)(a.batch)
+    tests = workloads.record() if a.workload == "record" else getattr(workloads, a.workload)(a.batch)
     b = kite.Batch()
     try:
         b.parts(a.parts)
@@ -117,7 +129,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--trial", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--workload", default="single_large")
-    ap.add_argument("--out", default=os.path.join(HERE, "tuned", "single_large.json"))
+    ap.add_argument("--out", default=None, help="result file (default: ahnum_amd/tuned/<workload>.json)")
     ap.add_argument("--seconds", type=float, default=3.0)
     ap.add_argument("--block", type=int, default=200)
     ap.add_argument("--variants", type=int, default=4, help="shape variants tried per length")
@@ -136,7 +148,10 @@ def main(argv=None):
               flush=True)
         return r
 
-    best = json.loads(json.dumps(DEFAULT))
+    if a.out is None:
+        a.out = os.path.join(HERE, "tuned", a.workload + ".json")
+    dflt_cfg = defaults(a.workload)
+    best = json.loads(json.dumps(dflt_cfg))
     base = measure(best, "defaults")
     best_tph = base["tests_per_hour"]
     for L in base["lengths"]:
@@ -147,7 +162,7 @@ def main(argv=None):
             if r["tests_per_hour"] > best_tph * (1 + a.margin / 100):
                 best, best_tph = cfg, r["tests_per_hour"]
     for key, values in (("shape_parts", [0, 1]), ("parts", [1, 2, 3, 4]), ("xcds", [1, 8]),
-                        ("batch", [int(v) for v in a.batches.split(",")])):
+                        ("batch", [] if a.workload in FIXED_BATCH else [int(v) for v in a.batches.split(",")])):
         for v in values:
             if v == best[key]:
                 continue
@@ -158,11 +173,11 @@ def main(argv=None):
                 best, best_tph = cfg, r["tests_per_hour"]
     # the winner against the defaults once more, each measured twice, so that one lucky trial
     # does not decide
-    if best != DEFAULT:
+    if best != dflt_cfg:
         again = [measure(best, "confirm best")["tests_per_hour"] for _ in range(2)]
-        dflt = [measure(json.loads(json.dumps(DEFAULT)), "confirm defaults")["tests_per_hour"] for _ in range(2)]
+        dflt = [measure(json.loads(json.dumps(dflt_cfg)), "confirm defaults")["tests_per_hour"] for _ in range(2)]
         if sum(again) <= sum(dflt) * (1 + a.margin / 100):
-            best = json.loads(json.dumps(DEFAULT))
+            best = json.loads(json.dumps(dflt_cfg))
         best_tph = max(sum(again), sum(dflt)) / 2
     result = {"workload": a.workload, "best": best, "tests_per_hour": best_tph,
               "defaults_tests_per_hour": base["tests_per_hour"], "seconds_per_trial": a.seconds,

@@ -141,7 +141,7 @@ def cpu_baseline(tests, secs):
 # ---- traffic summary ---------------------------------------------------------------------------
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_large_batch.txt")
-TUNED_FILE = os.path.join(ROOT, "ahnum_amd", "tuned", "single_large.json")
+TUNED_DIR = os.path.join(ROOT, "ahnum_amd", "tuned")  # <workload>.json: the batched autotuner's result
 
 
 def load_traffic(kernel):
@@ -279,9 +279,10 @@ def main():
     ap.add_argument("--batch", type=int, default=None,
                     help="tests of the single_large workload (default: the tuned value, else 384)")
     ap.add_argument("--tuned", default="auto",
-                    help="batched autotuner result (python -m ahnum_amd.tune_batch) applied to the single_large "
-                         "batch: shape variant per length, streams, XCD layout, batch size; 'auto' takes "
-                         "ahnum_amd/tuned/single_large.json when present, 'none' runs the defaults")
+                    help="batched autotuner result (python -m ahnum_amd.tune_batch) applied to the workload's "
+                         "batch: shape variant per length, shape parts, streams, XCD layout (and the batch size "
+                         "of single_large); 'auto' takes ahnum_amd/tuned/<workload>.json when present, 'none' "
+                         "runs the defaults")
     ap.add_argument("--workload", default="single_large", choices=sorted(WORKLOADS),
                     help="batch workload (single_large: the n~660K batch the metric is quoted on)")
     ap.add_argument("--plain-steps", type=int, default=10000, help="squarings of the one-test microbench")
@@ -337,8 +338,8 @@ def main():
     # the batched autotuner's settings; the shape variants reach the library through its tuning
     # file, which it reads once, so they are set before it loads
     tuned_path, tuned = None, {}
-    if a.workload == "single_large" and a.tuned != "none":
-        tuned_path = TUNED_FILE if a.tuned == "auto" else a.tuned
+    if a.tuned != "none":
+        tuned_path = os.path.join(TUNED_DIR, a.workload + ".json") if a.tuned == "auto" else a.tuned
         if os.path.exists(tuned_path):
             with open(tuned_path) as f:
                 tuned = json.load(f)["best"]
