@@ -20,6 +20,7 @@
 //
 // Work is described by per-work-group job records (slot, unit) over a table of slots, so the
 // same kernels run one test (kite.h API) or many tests of different lengths in one launch.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -559,10 +560,21 @@ void engine_copy(kite_ctx *ctx, void *dst, const void *src) {
         "hipMemcpyAsync(copy)");
 }
 
-static hipError_t launch_pass(PassKernel fn, int grid, size_t lds, hipStream_t st, const SlotDev *slots,
-                              const int2 *jobs, int mode, int step, int threads = kNT) {
-    void *args[] = {(void *)&slots, (void *)&jobs, (void *)&mode, (void *)&step};
+// With a start/stop event pair the launch goes through hipExtLaunchKernel, which stamps the events
+// from the dispatch packet itself (the kernel's own start and end, as a profiler's trace reads them),
+// not from marker packets of their own around it (those add the dispatch and completion latencies,
+// 5-7 us per launch).
+static hipError_t launch_k(const void *fn, int grid, int threads, void **args, size_t lds, hipStream_t st,
+                           hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    if (e0) return hipExtLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st, e0, e1, 0);
     return hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st);
+}
+
+static hipError_t launch_pass(PassKernel fn, int grid, size_t lds, hipStream_t st, const SlotDev *slots,
+                              const int2 *jobs, int mode, int step, int threads = kNT, hipEvent_t e0 = nullptr,
+                              hipEvent_t e1 = nullptr) {
+    void *args[] = {(void *)&slots, (void *)&jobs, (void *)&mode, (void *)&step};
+    return launch_k(fn, grid, threads, args, lds, st, e0, e1);
 }
 
 __global__ void k_set_slot(SlotDev *dst, SlotDev v) {
@@ -1337,7 +1349,7 @@ static int persist_grid(kite_batch *b, const kite_batch::Group &g, int threads, 
     return P >= g.pmin && P > 0 ? P * g.nl : 0;
 }
 
-// One launch per group of the pass; with `timed`, each launch is bracketed by an event pair.
+// One launch per group of the pass; with `timed`, each launch carries a start/stop event pair.
 // In square mode a column group with a fused kernel runs it as pass 1 and skips pass 2.
 static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool timed = false) {
     const auto &groups = pass == 0 ? b->grows : b->gcols;
@@ -1355,13 +1367,14 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
         // a call with timing on runs every launch on the first stream, so that each timed launch
         // has the device to itself (every call starts after all streams' earlier work)
         const hipStream_t st = b->timing ? b->ps[0] : b->ps[g.part];
-        auto launch = [&]() -> hipError_t {
-            if (!persist) return launch_pass(fn, g.count, lds, st, b->d_slots, jobs + g.off, mode, step, threads);
+        auto launch = [&](hipEvent_t e0, hipEvent_t e1) -> hipError_t {
+            if (!persist)
+                return launch_pass(fn, g.count, lds, st, b->d_slots, jobs + g.off, mode, step, threads, e0, e1);
             const int2 *pj = (pass == 0 ? b->d_rows : b->d_cols_p) + g.off;
             const unsigned int *ls = b->d_lists + (size_t)g.ls * kPersistListWords;
             int nl = g.nl;
             void *args[] = {(void *)&b->d_slots, (void *)&pj, (void *)&step, (void *)&ls, (void *)&nl};
-            return hipLaunchKernel(g.kp, dim3(pgrid), dim3(threads), args, lds, st);
+            return launch_k(g.kp, pgrid, threads, args, lds, st, e0, e1);
         };
         hipError_t e;
         if (timed) {
@@ -1375,11 +1388,9 @@ static hipError_t batch_launch(kite_batch *b, int pass, int mode, int step, bool
             // algorithmic bytes: each test's residue (8 L) read and written by the row pass and the
             // fused column pass, read or written by each split column pass
             b->ev_bytes.push_back((pass == 0 || fused ? 16.0 : 8.0) * g.sumL);
-            hipEventRecord(b->ev[ei], st);
-            e = launch();
-            hipEventRecord(b->ev[ei + 1], st);
+            e = launch(b->ev[ei], b->ev[ei + 1]);
         } else {
-            e = launch();
+            e = launch(nullptr, nullptr);
         }
         if (e != hipSuccess) return e;
     }

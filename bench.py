@@ -140,15 +140,21 @@ def cpu_baseline(tests, secs):
 
 # ---- traffic summary ---------------------------------------------------------------------------
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_large_batch.txt")
+# the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of each workload's benchmark command
+TRAFFIC_FILES = {w: os.path.join(ROOT, "profiles", f"hbm_traffic_{f}.txt")
+                 for w, f in (("single_large", "single_large_batch"), ("large_fft", "large_fft_batch"),
+                              ("record", "record"))}
 TUNED_DIR = os.path.join(ROOT, "ahnum_amd", "tuned")  # <workload>.json: the batched autotuner's result
 
 
-def load_traffic(kernel):
+def load_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    of this benchmark (lines "traffic <kernel> <bytes per launch>"), or None."""
+    of the workload's benchmark (lines "traffic <kernel> <bytes per launch>", given for kernels that
+    run one launch shape), or None."""
+    if workload not in TRAFFIC_FILES:
+        return None
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[workload]) as f:
             for line in f:
                 w = line.split()
                 if len(w) == 3 and w[0] == "traffic" and w[1] == kernel:
@@ -297,7 +303,7 @@ def main():
     ap.add_argument("--serve-leg-count", type=int, default=200, help="requests of the serving leg")
     ap.add_argument("--serve-leg-slots", type=int, default=16, help="jobs in flight at most in the serving leg")
     ap.add_argument("--time-every", type=int, default=10,
-                    help="bracket the launches of every N-th step of the kernel-timing region with HIP events")
+                    help="time the launches of every N-th step of the kernel-timing region with HIP events stamped from their dispatch packets")
     ap.add_argument("--time-steps", type=int, default=1000, help="steps of the kernel-timing region")
     ap.add_argument("--block", type=int, default=1000, help="squarings per square call (a Gerbicz block)")
     ap.add_argument("--parts", type=int, default=None,
@@ -424,7 +430,7 @@ def main():
     wall = time.perf_counter() - t0
     errs = b.maxerr()
     # timed region 2 (the roofline): the same squarings launched directly, the launches of every
-    # `time_every`-th step bracketed by HIP events on the batch stream
+    # `time_every`-th step timed by HIP events stamped from their own dispatch packets (hipExtLaunchKernel)
     b.parts(1)  # kernels alone, so that each launch's duration is its own
     b.timing(a.time_every)
     b.square(a.time_steps)
@@ -441,7 +447,7 @@ def main():
     avg_s = kms / 1e3 / klaunch
     alg_bytes = kbytes / klaunch
     achieved = alg_bytes / avg_s / 1e9
-    traffic = load_traffic(kname) if a.workload == "single_large" else None
+    traffic = load_traffic(a.workload, kname)
     step_bytes = sum(32 * L for L in Ls)
     step_flops = sum(5 * L * math.log2(L / 2) + 20 * L for L in Ls)
 

@@ -74,8 +74,9 @@ int kite_batch_maxerr(kite_batch *b, double *out, int reset);
 int kite_batch_sync(kite_batch *b);
 // Device time of each pass over the last kite_batch_square call timed with HIP events (ms):
 // out[0] rows, out[1] column inverse + carry, out[2] column forward. With timing every N > 0,
-// each launch of every N-th step is bracketed by events on the batch stream (0 turns it off);
-// launches3 counts the timed launches.
+// each launch of every N-th step carries a start/stop event pair stamped from its own dispatch
+// (hipExtLaunchKernel), on the batch's first stream (0 turns it off); launches3 counts the timed
+// launches.
 int kite_batch_set_timing(kite_batch *b, int every);
 int kite_batch_pass_times(kite_batch *b, double *out3, uint64_t *launches3);
 // Per-kernel totals over the same timed call: kernel i's name, summed device time (ms), number of
