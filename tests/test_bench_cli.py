@@ -44,3 +44,38 @@ def test_default_workload_is_the_metric_batch():
     tests = workloads.single_large(384, seed=seed)
     assert all(k % 2 == 1 and 1001 <= k <= 9999 and 650000 <= n <= 670000 for k, n in tests)
     assert "n~660K" in metric
+
+
+@pytest.mark.parametrize("name", ["single_large", "batch_mixed", "large_fft"])
+def test_tuned_files_fit_their_workload(name):
+    """Each committed batched-tuner result names its workload, keeps a fixed workload's test count,
+    sets only shape variants the library has (orders 0..3) for lengths the workload uses, and
+    beats the defaults it was measured against."""
+    import json
+    with open(os.path.join(bench.TUNED_DIR, name + ".json")) as f:
+        t = json.load(f)
+    assert t["workload"] == name
+    best = t["best"]
+    assert set(best) == {"orders", "shape_parts", "parts", "xcds", "batch"}
+    assert 1 <= best["parts"] <= 4 and best["xcds"] in (1, 8) and best["shape_parts"] in (0, 1)
+    wcount = bench.WORKLOADS[name][2]
+    if wcount is not None:
+        assert best["batch"] == wcount
+    lengths = {L for tr in t["trials"] for L in tr["lengths"]}
+    for L, o in best["orders"].items():
+        assert int(L) in lengths and 1 <= int(o) <= 3
+    assert t["tests_per_hour"] > t["defaults_tests_per_hour"]
+
+
+@pytest.mark.parametrize("workload,kernel", [
+    ("single_large", "k_cols_fused_r<96>"), ("record", "k_cols_fused_r<400>"), ("record", "k_rows_c<2048>"),
+])
+def test_traffic_summaries_give_the_dominant_kernels(workload, kernel):
+    """bench.py reads HBM bytes per launch from the committed rocprofv3 summaries: at least the
+    kernel's algorithmic bytes (16 L per test in the launch: the residue read and written; 307
+    tests at L = 49152 in the n~660K batch's main column launch) and at most 1.5 times them."""
+    b = bench.load_traffic(workload, kernel)
+    L, tests = {"single_large": (49152, 307), "record": (1638400, 1)}[workload]
+    alg = 16.0 * L * tests
+    assert b is not None and 1.0 <= b / alg <= 1.5
+    assert bench.load_traffic("plain_small", kernel) is None
