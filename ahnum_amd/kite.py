@@ -338,6 +338,7 @@ SIGNATURES.update({
 JOB_PRP, JOB_LLR, JOB_PENDING = 0, 1, 1
 ORDER_ARRIVAL, ORDER_LEAST_WORK = 0, 1
 SHARE_SQUARINGS, SHARE_TIME = 0, 1  # kite_run_opts.share
+LAUNCH_GRAPH, LAUNCH_DIRECT = 0, 1  # kite_run_opts.launch
 
 
 class Job(ctypes.Structure):
@@ -389,6 +390,7 @@ class RunOpts(ctypes.Structure):
         ("parts", c_u32),
         ("shape_parts", c_u32),
         ("xcds", c_u32),
+        ("launch", c_u32),
     ]
 
 
@@ -408,7 +410,7 @@ SIGNATURES.update({
 
 def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fault_job=0, fault_at=0,
               max_roundoff=0.0, on_done=None, checkpoint_dir=None, checkpoint_secs=0.0, order=ORDER_ARRIVAL,
-              share=SHARE_SQUARINGS, parts=0, shape_parts=0, xcds=0):
+              share=SHARE_SQUARINGS, parts=0, shape_parts=0, xcds=0, launch=0):
     """Runs jobs [(kind, k, n, arrival), ...] (kind JOB_PRP or JOB_LLR; Fermat base 3) through
     kite_batch_run, with checkpoint files in checkpoint_dir if given. Returns (rc, [JobResult])."""
     arr = (Job * len(jobs))(*[Job(k, n, kind, 3, 0, arrival) for kind, k, n, arrival in jobs])
@@ -426,6 +428,7 @@ def batch_run(jobs, slots=0, hbm_budget=0, block=1000, blocks=0, horizon=0.0, fa
     o.parts = parts
     o.shape_parts = shape_parts
     o.xcds = xcds
+    o.launch = launch
     if checkpoint_dir is not None:
         o.checkpoint_dir = os.fsencode(checkpoint_dir)
         o.checkpoint_secs = checkpoint_secs

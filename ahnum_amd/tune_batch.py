@@ -13,19 +13,23 @@ workload itself:
   batch    tests in flight (the rest of a larger workload would wait for a slot); single_large
            only: the other workloads are fixed sets of tests (batch_mixed 256, large_fft 32,
            plain_small 16), all in flight
+  graphs   square calls replayed from a captured HIP graph (true) or each step's kernels launched
+           directly on the streams (false)
 
 Each trial is a fresh process (the tuning file is read once per process) that squares the batch
 for `--seconds` after a warm-up, in square calls of `--block` steps replayed from the captured
-graph, and reports tests/hour. The search is coordinate-wise from the defaults (orders 0, 2
-parts, 8 XCDs, 384 tests): orders per length, then parts, XCDs and batch size, each kept only if it
+graph (or launched directly), and reports tests/hour. The search is coordinate-wise from the
+defaults (orders 0, 2 parts, 8 XCDs, 384 tests, graphs), or from a tuned file's settings
+(`--start FILE`), over every setting or those named (`--keys`): orders per length, then shape
+parts, parts, XCDs, batch size and graphs, each kept only if it
 measures more than `--margin` percent faster (shape parts are tried before the stream count, so that
 the count is tuned with them); the winner is then measured twice more against the
-defaults, and kept only if it still wins by the margin. The result is written as JSON (`--out`,
+defaults (or the start), and kept only if it still wins by the margin. The result is written as JSON (`--out`,
 default ahnum_amd/tuned/<workload>.json); bench.py applies the committed file of the workload it
 runs (or the one given with `--tuned FILE`).
 
 Usage: python -m ahnum_amd.tune_batch [--workload single_large|batch_mixed|large_fft] [--out FILE]
-       [--seconds 3]
+       [--seconds 3] [--start FILE] [--keys graphs,...]
 """
 import argparse
 import json
@@ -74,6 +78,7 @@ def trial_main(argv):
     ap.add_argument("--parts", type=int, default=2)
     ap.add_argument("--shape-parts", type=int, default=0)
     ap.add_argument("--xcds", type=int, default=8)
+    ap.add_argument("--graphs", type=int, default=1)
     ap.add_argument("--block", type=int, default=200)
     ap.add_argument("--seconds", type=float, default=3.0)
     a = ap.parse_args(argv)
@@ -85,6 +90,7 @@ def trial_main(argv):
         b.parts(a.parts)
         b.shape_parts(a.shape_parts)
         b.xcds(a.xcds)
+        b.graphs(bool(a.graphs))
         lengths = set()
         for i, (k, n) in enumerate(tests):
             s = b.add(k, n)
@@ -112,7 +118,7 @@ def trial_main(argv):
 def run_trial(cfg, workload, seconds, block):
     cmd = [sys.executable, "-m", "ahnum_amd.tune_batch", "--trial", "--workload", workload,
            "--batch", str(cfg["batch"]), "--parts", str(cfg["parts"]), "--xcds", str(cfg["xcds"]),
-           "--shape-parts", str(cfg.get("shape_parts", 0)),
+           "--shape-parts", str(cfg.get("shape_parts", 0)), "--graphs", str(int(cfg.get("graphs", True))),
            "--block", str(block), "--seconds", str(seconds)]
     env = apply_env(cfg)
     try:
@@ -125,6 +131,19 @@ def run_trial(cfg, workload, seconds, block):
     return json.loads(out.stdout.strip().splitlines()[-1])
 
 
+def merge_refinement(start, result):
+    """A search started from a tuned file (--start) keeps that file's record: its trials come first,
+    the defaults' rate stays the one it was measured against, and its command is kept beside the
+    new one."""
+    merged = dict(result)
+    merged["trials"] = list(start.get("trials", [])) + list(result["trials"])
+    merged["defaults_tests_per_hour"] = start.get("defaults_tests_per_hour", result["defaults_tests_per_hour"])
+    merged["commands"] = list(start.get("commands", [start["command"]] if "command" in start else [])) + \
+        [result["command"]]
+    merged.pop("command", None)
+    return merged
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--trial", action="store_true", help=argparse.SUPPRESS)
@@ -134,6 +153,11 @@ def main(argv=None):
     ap.add_argument("--block", type=int, default=200)
     ap.add_argument("--variants", type=int, default=4, help="shape variants tried per length")
     ap.add_argument("--batches", default="256,320,384,448")
+    ap.add_argument("--start", default=None,
+                    help="a tuned file whose best settings the search starts from (default: the defaults)")
+    ap.add_argument("--keys", default=None,
+                    help="comma-separated settings to search (orders, shape_parts, parts, xcds, batch, graphs; "
+                         "default: all)")
     ap.add_argument("--margin", type=float, default=1.0,
                     help="percent a setting must gain to replace the current one (trial noise is ~0.5%%)")
     a, rest = ap.parse_known_args(argv)
@@ -151,10 +175,16 @@ def main(argv=None):
     if a.out is None:
         a.out = os.path.join(HERE, "tuned", a.workload + ".json")
     dflt_cfg = defaults(a.workload)
+    start = None
+    if a.start:
+        with open(a.start) as f:
+            start = json.load(f)
+        dflt_cfg = start["best"]
+    keys = set(a.keys.split(",")) if a.keys else {"orders", "shape_parts", "parts", "xcds", "batch", "graphs"}
     best = json.loads(json.dumps(dflt_cfg))
-    base = measure(best, "defaults")
+    base = measure(best, "start" if a.start else "defaults")
     best_tph = base["tests_per_hour"]
-    for L in base["lengths"]:
+    for L in (base["lengths"] if "orders" in keys else []):
         for o in range(1, a.variants):
             cfg = json.loads(json.dumps(best))
             cfg["orders"][str(L)] = o
@@ -162,9 +192,12 @@ def main(argv=None):
             if r["tests_per_hour"] > best_tph * (1 + a.margin / 100):
                 best, best_tph = cfg, r["tests_per_hour"]
     for key, values in (("shape_parts", [0, 1]), ("parts", [1, 2, 3, 4]), ("xcds", [1, 8]),
-                        ("batch", [] if a.workload in FIXED_BATCH else [int(v) for v in a.batches.split(",")])):
+                        ("batch", [] if a.workload in FIXED_BATCH else [int(v) for v in a.batches.split(",")]),
+                        ("graphs", [True, False])):
+        if key not in keys:
+            continue
         for v in values:
-            if v == best[key]:
+            if v == best.get(key, True if key == "graphs" else None):
                 continue
             cfg = json.loads(json.dumps(best))
             cfg[key] = v
@@ -183,6 +216,8 @@ def main(argv=None):
               "defaults_tests_per_hour": base["tests_per_hour"], "seconds_per_trial": a.seconds,
               "block": a.block, "trials": trials,
               "command": "python -m ahnum_amd.tune_batch " + " ".join(sys.argv[1:] if argv is None else argv)}
+    if start is not None:
+        result = merge_refinement(start, result)
     os.makedirs(os.path.dirname(a.out), exist_ok=True)
     with open(a.out, "w") as f:
         json.dump(result, f, indent=1)

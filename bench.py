@@ -311,6 +311,9 @@ def main():
     ap.add_argument("--shape-parts", type=int, default=None,
                     help="each FFT shape but the most common on a stream of its own (default: the tuned value, "
                          "else 0)")
+    ap.add_argument("--launch", default=None, choices=["graph", "direct"],
+                    help="square calls replayed from a captured HIP graph or launched directly "
+                         "(default: the tuned file's choice, else graph)")
     ap.add_argument("--xcds", type=int, default=None,
                     help="XCDs the column jobs are laid out for (default: the tuned value, else 8)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -360,6 +363,8 @@ def main():
         a.parts = int(tuned.get("parts", 2))
     if a.xcds is None:
         a.xcds = int(tuned.get("xcds", 8))
+    if a.launch is None:
+        a.launch = "graph" if tuned.get("graphs", True) else "direct"
 
     from ahnum_amd import kite, workloads
 
@@ -402,17 +407,19 @@ def main():
     parts = b.parts(a.parts)
     shape_parts = b.shape_parts(int(tuned.get("shape_parts", 0)) if a.shape_parts is None else a.shape_parts)
     b.xcds(a.xcds)
+    b.graphs(a.launch == "graph")
     Ls = []
     for i, (k, n) in enumerate(tests):
         s = b.add(k, n)
         b.set(s, 0, workloads.random_residue_limbs(k * (1 << n) - 1, 100 + i))
         Ls.append(b.fft_len(s))
     # timed region 1 (the value): square calls of `block` steps, as the Gerbicz loop makes them,
-    # replayed from the captured HIP graph. Before it: the capture, one full warm replay of the same
-    # call, and replays until at least W steps and a quarter second of squaring have run (the
-    # device's clocks come up under load; a short W alone would time a cold device)
+    # replayed from the captured HIP graph or launched directly (--launch). Before it: the capture
+    # (with graphs), one full warm call of the same size, and calls until at least W steps and a
+    # quarter second of squaring have run (the device's clocks come up under load; a short W alone
+    # would time a cold device)
     block = min(a.block, a.steps)
-    b.square(block)  # captures the graph (and runs it)
+    b.square(block)  # captures the graph, if any (and runs it)
     warm, w0 = block, time.perf_counter()
     while warm < max(a.warmup, 2 * block) or time.perf_counter() - w0 < 0.25:
         b.square(block)
@@ -469,7 +476,9 @@ def main():
             "workload": described,
             "batch": len(tests),
             "fft_lengths": sorted(set(Ls)),
-            "launch": f"square calls of {block} steps replayed from a captured HIP graph, {parts} streams"
+            "launch": f"square calls of {block} steps "
+                      + ("replayed from a captured HIP graph" if a.launch == "graph" else "launched directly")
+                      + f", {parts} streams"
                       + (" for the most common FFT shape and one for each other" if shape_parts else "")
                       + f", column jobs laid out for {a.xcds} XCDs",
             "tuned": ({"file": os.path.relpath(tuned_path, ROOT), **tuned} if tuned_path else None),
@@ -513,7 +522,8 @@ def main():
     }
     if a.gerbicz_secs > 0:
         g = gerbicz_leg(tests, a.gerbicz_secs, a.gerbicz_blocks,
-                        {"parts": a.parts, "shape_parts": shape_parts, "xcds": a.xcds})
+                        {"parts": a.parts, "shape_parts": shape_parts, "xcds": a.xcds,
+                         "launch": 0 if a.launch == "graph" else 1})
         g["vs_value"] = round(g["tests_per_hour"] / out["value"], 4) if out["value"] else None
         out["gerbicz_checked"] = g
     if a.serve_secs > 0 and a.workload == "single_large":

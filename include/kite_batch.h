@@ -56,7 +56,9 @@ int kite_batch_set_shape_parts(kite_batch *b, int on);
 // setting in use.
 int kite_batch_set_persistent(kite_batch *b, int on);
 // Square calls of more than one step are captured as a HIP graph and replayed while the batch's
-// slots are unchanged (on by default; the launches are the same either way).
+// slots are unchanged (on by default; the launches are the same either way). Off, each step's
+// kernels are launched directly, on streams that each have a hardware queue of their own (created
+// at the first switch; the batch's work so far is waited for). Returns 0, -1 on a HIP failure.
 int kite_batch_set_graphs(kite_batch *b, int on);
 // Squarings of this slot become x -> x^2 + xadd (|xadd| < 2^20; 0 by default, -2 for LLR steps).
 int kite_batch_set_xadd(kite_batch *b, int slot, int32_t xadd);
@@ -136,6 +138,7 @@ enum { KITE_JOB_PRP = 0, KITE_JOB_LLR = 1 };
 enum { KITE_JOB_PENDING = 1 };  // status of a job the horizon stopped before it finished
 enum { KITE_ORDER_ARRIVAL = 0, KITE_ORDER_LEAST_WORK = 1 };  // kite_run_opts.order
 enum { KITE_SHARE_SQUARINGS = 0, KITE_SHARE_TIME = 1 };      // kite_run_opts.share
+enum { KITE_LAUNCH_GRAPH = 0, KITE_LAUNCH_DIRECT = 1 };       // kite_run_opts.launch
 
 typedef struct {
     uint64_t k, n;
@@ -202,6 +205,11 @@ typedef struct {
     uint32_t parts;
     uint32_t shape_parts;
     uint32_t xcds;
+    // How a round's squarings reach the device: KITE_LAUNCH_GRAPH (replayed from a captured HIP
+    // graph, as kite_batch_set_graphs(b, 1)) or KITE_LAUNCH_DIRECT (each step's kernels launched
+    // on the batch's streams, as kite_batch_set_graphs(b, 0)). Which runs faster depends on the
+    // workload's launches; the batched tuner measures both.
+    uint32_t launch;
 } kite_run_opts;
 
 // Device bytes a job of this kind holds at length L while in flight (the admission estimate).

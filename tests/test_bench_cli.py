@@ -49,15 +49,16 @@ def test_default_workload_is_the_metric_batch():
 @pytest.mark.parametrize("name", ["single_large", "batch_mixed", "large_fft"])
 def test_tuned_files_fit_their_workload(name):
     """Each committed batched-tuner result names its workload, keeps a fixed workload's test count,
-    sets only shape variants the library has (orders 0..3) for lengths the workload uses, and
-    beats the defaults it was measured against."""
+    sets only shape variants the library has (orders 0..3) for lengths the workload uses, chooses
+    graph replay or direct launches, and beats the defaults it was measured against."""
     import json
     with open(os.path.join(bench.TUNED_DIR, name + ".json")) as f:
         t = json.load(f)
     assert t["workload"] == name
     best = t["best"]
-    assert set(best) == {"orders", "shape_parts", "parts", "xcds", "batch"}
+    assert set(best) - {"graphs"} == {"orders", "shape_parts", "parts", "xcds", "batch"}
     assert 1 <= best["parts"] <= 4 and best["xcds"] in (1, 8) and best["shape_parts"] in (0, 1)
+    assert best.get("graphs", True) in (True, False)
     wcount = bench.WORKLOADS[name][2]
     if wcount is not None:
         assert best["batch"] == wcount
@@ -79,3 +80,20 @@ def test_traffic_summaries_give_the_dominant_kernels(workload, kernel):
     alg = 16.0 * L * tests
     assert b is not None and 1.0 <= b / alg <= 1.5
     assert bench.load_traffic("plain_small", kernel) is None
+
+
+def test_tuner_refinement_keeps_the_start_record():
+    """A search started from a tuned file (--start) keeps that file's trials first, the rate of the
+    defaults it was measured against and its command beside the new one."""
+    from ahnum_amd import tune_batch
+    start = {"workload": "w", "best": {"parts": 2}, "tests_per_hour": 10.0, "defaults_tests_per_hour": 8.0,
+             "trials": [{"tried": "defaults"}, {"tried": "parts -> 2"}], "command": "python -m ahnum_amd.tune_batch"}
+    new = {"workload": "w", "best": {"parts": 2, "graphs": False}, "tests_per_hour": 11.0,
+           "defaults_tests_per_hour": 10.0, "trials": [{"tried": "start"}, {"tried": "graphs -> False"}],
+           "command": "python -m ahnum_amd.tune_batch --start f --keys graphs"}
+    m = tune_batch.merge_refinement(start, new)
+    assert [t["tried"] for t in m["trials"]] == ["defaults", "parts -> 2", "start", "graphs -> False"]
+    assert m["defaults_tests_per_hour"] == 8.0 and m["tests_per_hour"] == 11.0
+    assert m["commands"] == [start["command"], new["command"]] and "command" not in m
+    again = tune_batch.merge_refinement(m, dict(new, command="second"))
+    assert again["commands"] == [start["command"], new["command"], "second"]

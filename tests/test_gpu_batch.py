@@ -164,11 +164,13 @@ def test_batch_llr_matches_prst(golden):
             assert r.confirm_runs == 1  # a composite is confirmed once at the next length
 
 
-def test_batch_mixed_fermat_and_llr(golden):
-    """Fermat and LLR jobs sharing launches; each result equals the known one."""
+@pytest.mark.parametrize("launch", [kite.LAUNCH_GRAPH, kite.LAUNCH_DIRECT])
+def test_batch_mixed_fermat_and_llr(golden, launch):
+    """Fermat and LLR jobs sharing launches, rounds replayed from graphs or launched directly
+    (kite_run_opts.launch); each result equals the known one."""
     jobs = [(kite.JOB_PRP, 9357, 20000, 0.0), (kite.JOB_LLR, 9357, 20000, 0.0), (kite.JOB_PRP, 8191, 25000, 0.0),
             (kite.JOB_LLR, 3, 26459, 0.0), (kite.JOB_PRP, 3, 41628, 0.0), (kite.JOB_LLR, 33, 99999, 0.0)]
-    rc, out = kite.batch_run(jobs, block=1000)
+    rc, out = kite.batch_run(jobs, block=1000, launch=launch)
     assert rc == 0
     assert "%016X" % out[0].res64 == "79291CD2F47CE48C" and not out[0].is_prime
     assert "%016X" % out[1].res64 == "DC52061161A0DE1C" and not out[1].is_prime
