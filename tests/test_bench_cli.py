@@ -46,7 +46,7 @@ def test_default_workload_is_the_metric_batch():
     assert "n~660K" in metric
 
 
-@pytest.mark.parametrize("name", ["single_large", "batch_mixed", "large_fft"])
+@pytest.mark.parametrize("name", ["single_large", "batch_mixed", "large_fft", "plain_small"])
 def test_tuned_files_fit_their_workload(name):
     """Each committed batched-tuner result names its workload, keeps a fixed workload's test count,
     sets only shape variants the library has (orders 0..3) for lengths the workload uses, chooses
