@@ -12,7 +12,7 @@ workload itself:
   xcds     XCDs the column jobs are laid out for (1: slot order)
   batch    tests in flight (the rest of a larger workload would wait for a slot); single_large
            only: the other workloads are fixed sets of tests (batch_mixed 256, large_fft 32,
-           plain_small 16), all in flight
+           plain_small 16, record 1), all in flight
   graphs   square calls replayed from a captured HIP graph (true) or each step's kernels launched
            directly on the streams (false)
 
@@ -43,7 +43,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 DEFAULT = {"orders": {}, "shape_parts": 0, "parts": 2, "xcds": 8, "batch": 384}
 # workloads that are a fixed set of tests (all in flight): their batch is the set's size, not tuned
-FIXED_BATCH = {"batch_mixed": 256, "large_fft": 32, "plain_small": 16}
+FIXED_BATCH = {"batch_mixed": 256, "large_fft": 32, "plain_small": 16, "record": 1}
 
 
 def defaults(workload):
