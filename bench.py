@@ -172,7 +172,7 @@ def pct(values, p):
 
 
 def run_serve(count, rate, n_lo, n_hi, horizon, hbm_gb, slots, order_name, ckpt_dir="", ckpt_secs=0.0,
-              verbose=False, share_name="squarings"):
+              verbose=False, share_name="squarings", launch_name="graph"):
     """A finite Poisson stream of Fermat and LLR requests through kite_batch_run; the serving record."""
     from ahnum_amd import kite, workloads
     stream = workloads.serve_stream(count=count, rate=rate, n_lo=n_lo, n_hi=n_hi)
@@ -189,7 +189,8 @@ def run_serve(count, rate, n_lo, n_hi, horizon, hbm_gb, slots, order_name, ckpt_
     t0 = time.perf_counter()
     rc, out = kite.batch_run(jobs, slots=slots, hbm_budget=int(hbm_gb * 2**30), block=1000, horizon=horizon,
                              on_done=progress, checkpoint_dir=ckpt_dir or None, checkpoint_secs=ckpt_secs,
-                             order=order, share=kite.SHARE_TIME if share_name == "time" else kite.SHARE_SQUARINGS)
+                             order=order, share=kite.SHARE_TIME if share_name == "time" else kite.SHARE_SQUARINGS,
+                             launch=kite.LAUNCH_DIRECT if launch_name == "direct" else kite.LAUNCH_GRAPH)
     wall = time.perf_counter() - t0
     span = min(wall, horizon) if horizon else wall
     arrived = sum(1 for j in jobs if j[3] <= span)
@@ -200,6 +201,7 @@ def run_serve(count, rate, n_lo, n_hi, horizon, hbm_gb, slots, order_name, ckpt_
     rec = {
         "requests": count, "rate_per_s": rate, "n_range": [n_lo, n_hi], "horizon_s": horizon,
         "hbm_budget_gib": hbm_gb, "slots": slots, "order": order_name, "share": share_name, "block": 1000,
+        "launch": launch_name,
         "arrived": arrived,
         "completed": {"fermat": len(lat["fermat"]), "llr": len(lat["llr"])},
         "completed_tests_per_hour": round(len(tested) * 3600.0 / span, 1),
@@ -224,7 +226,7 @@ def run_serve(count, rate, n_lo, n_hi, horizon, hbm_gb, slots, order_name, ckpt_
 def serve_only(a):
     rec = run_serve(a.serve_count, a.serve_rate, a.serve_n_lo, a.serve_n_hi, a.serve_horizon, a.serve_hbm_gb,
                     a.serve_slots, a.serve_order, a.serve_checkpoint_dir, a.serve_checkpoint_secs, verbose=True,
-                    share_name=a.serve_share)
+                    share_name=a.serve_share, launch_name=a.serve_launch)
     line = {
         "metric": "serving: completed tests/hour (Poisson arrivals, 50% Fermat + 50% LLR, n log-uniform)",
         "value": rec["completed_tests_per_hour"],
@@ -331,6 +333,9 @@ def main():
                     help="admission order among arrived requests")
     ap.add_argument("--serve-share", default="time", choices=["squarings", "time"],
                     help="how jobs in flight share the rounds (kite_run_opts.share)")
+    ap.add_argument("--serve-launch", default="direct", choices=["graph", "direct"],
+                    help="rounds of the serving runs replayed from captured graphs or launched directly "
+                         "(direct: what the batched tuner chose for every tuned workload)")
     ap.add_argument("--serve-checkpoint-dir", default="",
                     help="jobs resume from and write checkpoints here; the horizon saves every job in flight")
     ap.add_argument("--serve-checkpoint-secs", type=float, default=0.0, help="checkpoint interval (0 = 600 s)")
@@ -529,7 +534,8 @@ def main():
     if a.serve_secs > 0 and a.workload == "single_large":
         out["serve"] = run_serve(count=a.serve_leg_count, rate=a.serve_rate, n_lo=a.serve_n_lo,
                                  n_hi=a.serve_n_hi, horizon=a.serve_secs, hbm_gb=a.serve_hbm_gb,
-                                 slots=a.serve_leg_slots, order_name="least_work", share_name="time")
+                                 slots=a.serve_leg_slots, order_name="least_work", share_name="time",
+                                 launch_name=a.serve_launch)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tests[:16], a.cpu_secs)
     print(json.dumps(out), flush=True)
