@@ -1068,7 +1068,8 @@ extern "C" int kite_batch_run(const kite_job *jobs, size_t njobs, const kite_run
     if (o.parts) kite_batch_set_parts(S.b, (int)o.parts);
     if (o.shape_parts) kite_batch_set_shape_parts(S.b, 1);
     if (o.xcds) kite_batch_set_xcds(S.b, (int)o.xcds);
-    if (o.launch == KITE_LAUNCH_DIRECT) kite_batch_set_graphs(S.b, 0);
+    // a failure here is latched in the batch: the first round then fails and the run reports it
+    if (o.launch == KITE_LAUNCH_DIRECT) (void)kite_batch_set_graphs(S.b, 0);
     S.max_slots = o.slots ? o.slots : (size_t)-1;
     S.hbm_budget = o.hbm_budget;
     S.order = o.order;

@@ -525,8 +525,10 @@ class Batch:
         return self.lib.kite_batch_set_xcds(self.p, int(n))
 
     def graphs(self, on):
-        """Replay multi-step square calls from a captured HIP graph (the default) or launch directly."""
-        self.lib.kite_batch_set_graphs(self.p, int(bool(on)))
+        """Replay multi-step square calls from a captured HIP graph (the default) or launch directly
+        (on streams with hardware queues of their own); switching waits for the batch's work."""
+        if self.lib.kite_batch_set_graphs(self.p, int(bool(on))):
+            raise RuntimeError("kite_batch_set_graphs failed (a HIP call failed)")
 
     def shape_parts(self, on):
         """Each FFT shape but the most common on a stream of its own (kite_batch_set_shape_parts)."""
