@@ -298,6 +298,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gerbicz-secs", type=float, default=30.0,
                     help="length of the Gerbicz-checked leg (0: skip)")
+    ap.add_argument("--gerbicz-launch", default="graph", choices=["graph", "direct"],
+                    help="rounds of the Gerbicz-checked loop replayed from captured graphs or launched directly "
+                         "(graph: measured faster for this loop; the squaring region follows the tuned file)")
     ap.add_argument("--gerbicz-blocks", type=int, default=64,
                     help="blocks of 1000 squarings between checks in the Gerbicz-checked leg (a check "
                          "costs one block: 64 keeps it at 1.6%%)")
@@ -528,7 +531,8 @@ def main():
     if a.gerbicz_secs > 0:
         g = gerbicz_leg(tests, a.gerbicz_secs, a.gerbicz_blocks,
                         {"parts": a.parts, "shape_parts": shape_parts, "xcds": a.xcds,
-                         "launch": 0 if a.launch == "graph" else 1})
+                         "launch": 0 if a.gerbicz_launch == "graph" else 1})
+        g["launch"] = a.gerbicz_launch
         g["vs_value"] = round(g["tests_per_hour"] / out["value"], 4) if out["value"] else None
         out["gerbicz_checked"] = g
     if a.serve_secs > 0 and a.workload == "single_large":
